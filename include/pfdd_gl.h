@@ -1,0 +1,171 @@
+/*
+ * pfdd_gl.h -- C ABI of the MI355X (gfx950) Ginzburg-Landau phase-field solver.
+ *
+ * One handle owns one periodic grid of `ns` complex order parameters (one per
+ * slip system) resident on one GPU, the packed Green's-function matrix B(k)
+ * and the work buffers of the GL step.  Plain pointers and sizes only; every
+ * function returns 0 (PFDD_OK) or an error code, with a message available from
+ * pfdd_gl_last_error() on the calling thread.
+ *
+ * Each entry point names the reference interface it replaces
+ * (lanl/Phase-Field-Dislocation-Dynamics-PFDD-2.x, src/ directory).
+ *
+ * Host layout of a field is the reference's `fft->xi[0]`:
+ *   xi[2*(k + j*nz + i*ny*nz + a*nx*ny*nz) + {0: re, 1: im}]  (float64)
+ *
+ * Threading: a handle is used from one host thread at a time; all its work is
+ * ordered on one HIP stream.  Different handles are independent.
+ */
+#ifndef PFDD_GL_H
+#define PFDD_GL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFDD_GL_MAX_SYSTEMS 12
+#define PFDD_GL_MAX_COEF 8
+
+enum pfdd_gl_status {
+    PFDD_OK = 0,
+    PFDD_ERR_ARG = 1,         /* invalid argument */
+    PFDD_ERR_UNSUPPORTED = 2, /* grid size / system count not built in */
+    PFDD_ERR_HIP = 3,         /* HIP runtime error */
+    PFDD_ERR_NONFINITE = 4,   /* a reduction produced NaN/Inf (field diverged) */
+    PFDD_ERR_STATE = 5        /* call out of order */
+};
+
+enum pfdd_gl_precision { PFDD_FP64 = 0, PFDD_FP32 = 1 };
+
+/* Core-energy models (App::core_energy, app.cpp:663-672).  All coefficients
+ * are already divided by mu*dslip*b, as the reference does inside each
+ * core_energy_* routine. */
+enum pfdd_gl_core {
+    PFDD_CORE_NONE = 0,
+    PFDD_CORE_PERFECT = 1, /* flag 1, fft_fftw_slab.cpp:1130-1191: coef[0] = An'            */
+    PFDD_CORE_SINE = 3,    /* flag 3, :1360-1418: coef[0] = isf', coef[1] = Cn'             */
+    PFDD_CORE_MPEA = 7,    /* flag 7, :1196-1235: amp[] = A_mpea/(mu dslip b) per cell      */
+    PFDD_CORE_SERIES = 8   /* sine series sum_m c_{a,m} sin^2(m pi x), optional amp[]:
+                              coef[a*ncoef + m-1], generalises flags 1/3/7 (and 6, whose
+                              An' the host recomputes per stress state)                   */
+};
+
+typedef struct pfdd_gl pfdd_gl;
+
+/* Everything FFTW_Slab::init/setup (fft_fftw_slab.cpp:52-119) and the app's
+ * set_slip/constructor supply to the GL loop. */
+typedef struct pfdd_gl_desc {
+    int nx, ny, nz;      /* grid: fft->nx, ny, nz (create_sites.cpp:228-232)           */
+    int ns;              /* slip systems: app->slip_systems                             */
+    int precision;       /* PFDD_FP64 (plain) or PFDD_FP32                              */
+    double four[9];      /* reciprocal matrix fft->four, row-major (app.cpp:396-431)    */
+    double C44, C12, C11;/* material stiffness (material.cpp:156-160)                  */
+    double S12, S44;     /* material compliance (material.cpp:162-164), diagnostics     */
+    double dslip;        /* material->dslip                                             */
+    const double *xn;    /* ns*3 slip-plane normals, already rotated by set_slip        */
+    const double *xb;    /* ns*3 Burgers vectors, already rotated by set_slip           */
+    double cddt;         /* app->CD * app->timestep                                     */
+    int core_model;      /* enum pfdd_gl_core                                           */
+    int ncoef;           /* coefficients per system for PFDD_CORE_SERIES (<= 8)         */
+    const double *coef;  /* see enum pfdd_gl_core                                       */
+    const double *amp;   /* optional ns*N per-cell amplitude (flags 7, 8), may be NULL  */
+    const double *xo;    /* optional ns*N obstacle mask fft->xo (nonzero = frozen)      */
+    int device;          /* HIP device ordinal                                          */
+    void *stream;        /* hipStream_t to order all work on; NULL = handle's own       */
+} pfdd_gl_desc;
+
+/* Scalars of the last executed GL iteration (update_order_parameter,
+ * fft_fftw_slab.cpp:1746-1790, plus the energies of the pre-update state). */
+typedef struct pfdd_gl_reduce {
+    double xinorm;      /* fft->xinorm: sqrt(sum |dxi|^2), quirk Q3 under masks        */
+    double xiave;       /* fft->xiave: sum over systems and cells of Re xi (a sum)     */
+    double e_elastic;   /* (1/2N) sum_k xihat^* B xihat                                 */
+    double e_core;      /* sum_r sum_a core energy density                              */
+    double e_stress;    /* sum_a tau_a sum_r Re xi_a                                    */
+    double e_total;     /* e_elastic + e_core - e_stress                                */
+    double e_core_ref;  /* fft->E_core as the reference accumulates it (sum / nx)       */
+    int steps_done;     /* GL iterations executed by the last pfdd_gl_step call        */
+    int converged;      /* 1 if it stopped because xinorm < tol                         */
+} pfdd_gl_reduce;
+
+/* Library version, (major << 16) | minor. */
+int pfdd_gl_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *pfdd_gl_last_error(void);
+
+/* Replaces FFTW_Slab::init + allocate + setup (fft_fftw_slab.cpp:52-237,
+ * 111-119): allocates device fields, builds B(k) (Bmatrix, :466-638) on the
+ * device in FP64.  The initial xi is zero. */
+int pfdd_gl_create(const pfdd_gl_desc *desc, pfdd_gl **out);
+
+/* Frees everything the handle owns.  (The reference never frees its FFT
+ * arrays: ~FFTW_Slab is empty, fft_fftw_slab.cpp:43-46.) */
+int pfdd_gl_destroy(pfdd_gl *h);
+
+/* Upload / download the order parameter in the host layout of fft->xi[0]
+ * (the reference's initial_sxtal_* writers and compute_* readers). */
+int pfdd_gl_set_xi(pfdd_gl *h, const double *xi_host);
+int pfdd_gl_get_xi(pfdd_gl *h, double *xi_host);
+
+/* Device-resident access: pointer to the current field on the device (layout
+ * as above, element type double2 for FP64 or float2 for FP32).  The pointer
+ * changes after every GL iteration (ping-pong buffers); fetch it again. */
+int pfdd_gl_xi_device(pfdd_gl *h, void **dptr);
+/* Copy a device buffer (same element type/layout) into the current field. */
+int pfdd_gl_set_xi_device(pfdd_gl *h, const void *dptr);
+
+/* Resolved shear stresses tau_a in units of mu: the result of
+ * App::resolSS -> FFTW_Slab::resolSS_Schmid (fft_fftw_slab.cpp:737-769). */
+int pfdd_gl_set_tau(pfdd_gl *h, const double *tau);
+
+/* Replace the core-energy coefficients (same meaning as in the descriptor),
+ * e.g. per stress state for flag 6 (core_energy_USFE_angle_tau, :1500-1545). */
+int pfdd_gl_set_core_coef(pfdd_gl *h, int ncoef, const double *coef);
+
+/* Runs up to `nsteps` GL iterations at the current tau: the inner loop of
+ * SolveGL::iterate (solve_GL.cpp:86-148) -- core_energy, prep_forward,
+ * internal_energy, prep_backward, update_order_parameter, prepare_next_itr --
+ * stopping (device-side, no host round trip) after the first iteration with
+ * xinorm < tol (tol <= 0: never).  If `out` is non-NULL the call waits for
+ * the GPU and reports the last executed iteration; otherwise it only
+ * enqueues the work. */
+int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out);
+
+/* Per-system sums of Re xi over all cells of the current field (used by the
+ * stress-free strain average, stressfree_strain :3252-3326). */
+int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re);
+
+/* Output-step averages: DiagStrain::compute + DiagStress::compute
+ * (diag_strain.cpp:44-61, diag_stress.cpp:43-73) i.e. strain(),
+ * total_average_strain(), stress() (fft_fftw_slab.cpp:3252-3553), for the
+ * current (post-update) field.  sigma_rot in units of mu; results in Pa-based
+ * units exactly as fft->ave_eps / fft->ave_sigma (row-major 3x3). */
+int pfdd_gl_diag(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
+                 double ave_sigma[9]);
+
+/* Same as pfdd_gl_diag but also returns the fields the dump command reads:
+ * data_eps and data_sigma (2*9*N doubles each, layout of fft->data_eps /
+ * fft->data_sigma: [2*(r + i*N + j*3N) + {re,im}]).  The strain field is built
+ * from the k-space field of the last step's pre-update xi, as the reference's
+ * output step does (strain() reads data_fftw); before any step it uses the
+ * current field.  Either output pointer may be NULL. */
+int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
+                        double ave_sigma[9], double *data_eps, double *data_sigma);
+
+/* Average device time (ms) of each kernel of one GL iteration over `nsteps`
+ * iterations, measured with HIP events on the handle's stream.  ms[0..4] =
+ * forward z pass, forward y pass, x pass with contraction, inverse y pass,
+ * inverse z pass with update; ms[5] = reduction; ms[6] = whole iteration.
+ * nkernels receives the number of kernels of one iteration. */
+int pfdd_gl_profile(pfdd_gl *h, int nsteps, double *ms, int *nkernels);
+
+/* Bytes the GL iteration moves, by the kernel split above (algorithmic
+ * figure: each field element read/written once per kernel, B once). */
+int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFDD_GL_H */

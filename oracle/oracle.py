@@ -1,0 +1,271 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/pfdd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the product.
+
+Array conventions follow the reference (lanl PFDD-2.x, src/fft_fftw_slab.cpp):
+``xi`` is complex128 of shape (S, nx, ny, nz) (bit-identical to the
+reference's interleaved ``xi[0][2*(k + j*nz + i*ny*nz + a*N) + {0,1}]``),
+``BB`` is float64 of shape (S_b, S_a, nx, ny, nz) i.e. flat index
+``r + a*N + b*N*S`` (fft_fftw_slab.cpp:631).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -O2 -fopenmp)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_double = ctypes.c_int, ctypes.c_double
+        sigs = {
+            "orc_set_threads": (None, [c_int]),
+            "orc_get_threads": (c_int, []),
+            "orc_material": (None, [c_double, c_double, _dp]),
+            "orc_voigt": (None, [c_double, c_double, c_double, _dp]),
+            "orc_prim": (None, [_dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_set_scale": (None, [_dp, c_double, c_double, c_double, _dp, _dp, _dp]),
+            "orc_rotate_slip": (None, [c_int, _dp, _dp, _dp, _dp]),
+            "orc_rotate_stress": (None, [_dp, _dp, c_double, _dp, _dp]),
+            "orc_resolss": (None, [c_int, _dp, _dp, _dp, c_double, _dp]),
+            "orc_frec": (None, [c_int, c_int, c_int, _dp, _dp, _dp, _dp]),
+            "orc_stiffness": (None, [c_double, c_double, _dp]),
+            "orc_bmatrix": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_double,
+                                   c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_fmatrix": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_double,
+                                   c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_fft3d": (None, [c_int, c_int, c_int, _dp, c_int]),
+            "orc_dft1d_naive": (None, [c_int, _dp, _dp, c_int]),
+            "orc_core_energy": (c_double, [c_int, c_int, c_int, c_int, _dp, c_int, _dp, c_int,
+                                           _dp, _dp, _dp]),
+            "orc_internal_energy": (None, [c_int, c_int, c_int, c_int, _dp, _dp, _dp]),
+            "orc_update": (None, [c_int, c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp,
+                                  c_double, _dp]),
+            "orc_gl_step": (None, [c_int, c_int, c_int, c_int, _dp, _dp, _dp, c_double, _dp,
+                                   c_int, _dp, c_int, _dp, _dp, _dp]),
+            "orc_diagnostics": (None, [c_int, c_int, c_int, c_int, _dp, _dp, _dp, _dp,
+                                       c_double, c_double, c_double, c_double, c_double,
+                                       c_double, _dp, _dp, _dp, _dp, _dp]),
+            "orc_ranmar_init": (None, [c_int, _dp]),
+            "orc_ranmar_uniform": (c_double, [_dp]),
+            "orc_ranmar_fill": (None, [_dp, ctypes.c_long, _dp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    if a.dtype == np.complex128:
+        return a.ctypes.data_as(_dp)
+    assert a.dtype == np.float64, a.dtype
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+# --------------------------------------------------------------- setup pieces
+def material(mu: float, young: float) -> dict:
+    out = np.zeros(8)
+    lib().orc_material(mu, young, _p(out))
+    return dict(zip(["C44", "C12", "C11", "nu", "S11", "S12", "S44", "ll"], out.tolist()))
+
+
+def voigt(C44, C12, C11) -> dict:
+    out = np.zeros(4)
+    lib().orc_voigt(C44, C12, C11, _p(out))
+    return dict(zip(["mu", "ll", "young", "xnu"], out.tolist()))
+
+
+def prim(x, y, z):
+    p, ip, nv = np.zeros(9), np.zeros(9), ctypes.c_double()
+    lib().orc_prim(_p(_f64(x)), _p(_f64(y)), _p(_f64(z)), _p(p), _p(ip), ctypes.byref(nv))
+    return p.reshape(3, 3), ip.reshape(3, 3), nv.value
+
+
+def set_scale(prim_m, sa=1.0, sb=1.0, sc=1.0):
+    four, nf, vol = np.zeros(9), np.zeros(3), ctypes.c_double()
+    lib().orc_set_scale(_p(_f64(prim_m).reshape(-1)), sa, sb, sc, _p(four), _p(nf),
+                        ctypes.byref(vol))
+    return four.reshape(3, 3), nf, vol.value
+
+
+def rotate_slip(xn, xb, four, normfour):
+    xn = _f64(xn).copy()
+    xb = _f64(xb).copy()
+    lib().orc_rotate_slip(xn.shape[0], _p(xn), _p(xb), _p(_f64(four).reshape(-1)), _p(_f64(normfour)))
+    return xn, xb
+
+
+def rotate_stress(prim_m, invprim, norvol, sigma):
+    out = np.zeros(9)
+    lib().orc_rotate_stress(_p(_f64(prim_m).reshape(-1)), _p(_f64(invprim).reshape(-1)), norvol,
+                            _p(_f64(sigma).reshape(-1)), _p(out))
+    return out.reshape(3, 3)
+
+
+def resolss(sigma_rot, xn, xb, dslip):
+    xn = _f64(xn)
+    tau = np.zeros(xn.shape[0])
+    lib().orc_resolss(xn.shape[0], _p(_f64(sigma_rot).reshape(-1)), _p(xn), _p(_f64(xb)), dslip, _p(tau))
+    return tau
+
+
+def frec(nx, ny, nz, four):
+    fx, fy, fz = (np.zeros((nx, ny, nz)) for _ in range(3))
+    lib().orc_frec(nx, ny, nz, _p(_f64(four).reshape(-1)), _p(fx), _p(fy), _p(fz))
+    return fx, fy, fz
+
+
+def stiffness(C44, C12):
+    C = np.zeros(81)
+    lib().orc_stiffness(C44, C12, _p(C))
+    return C.reshape(3, 3, 3, 3)
+
+
+def bmatrix(shape, C44, C12, C11, dslip, xn, xb, four):
+    """Returns (BB[S_b, S_a, nx, ny, nz], eps[S,3,3], moduli dict)."""
+    nx, ny, nz = shape
+    xn, xb = _f64(xn), _f64(xb)
+    ns = xn.shape[0]
+    fx, fy, fz = frec(nx, ny, nz, four)
+    BB = np.zeros((ns, ns, nx, ny, nz))
+    eps = np.zeros((ns, 3, 3))
+    mod = np.zeros(4)
+    lib().orc_bmatrix(nx, ny, nz, ns, C44, C12, C11, dslip, _p(xn), _p(xb), _p(fx), _p(fy),
+                      _p(fz), _p(BB), _p(eps), _p(mod))
+    return BB, eps, dict(zip(["mu", "ll", "young", "xnu"], mod.tolist()))
+
+
+def fmatrix(shape, C44, C12, mu, xnu, eps, four, with_D=False):
+    nx, ny, nz = shape
+    eps = _f64(eps)
+    ns = eps.shape[0]
+    fx, fy, fz = frec(nx, ny, nz, four)
+    FF = np.zeros((3, 3, ns, nx, ny, nz))  # [j][i][a][r]
+    DD = np.zeros_like(FF) if with_D else None
+    lib().orc_fmatrix(nx, ny, nz, ns, C44, C12, mu, xnu, _p(eps), _p(fx), _p(fy), _p(fz),
+                      _p(FF), _p(DD))
+    return (FF, DD) if with_D else FF
+
+
+def fft3d(x: np.ndarray, sign: int) -> np.ndarray:
+    """Unnormalised in-place-style 3-D DFT (FFTW 2 semantics) of a complex128 array."""
+    y = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    n0, n1, n2 = y.shape
+    lib().orc_fft3d(n0, n1, n2, _p(y), int(sign))
+    return y
+
+
+def dft1d_naive(x: np.ndarray, sign: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib().orc_dft1d_naive(x.shape[0], _p(x), _p(out), int(sign))
+    return out
+
+
+def core_energy(xi, flag, par, nm=0, amp=None):
+    S, nx, ny, nz = xi.shape
+    dE = np.zeros((S, nx, ny, nz))
+    ev = ctypes.c_double()
+    par = _f64(par) if par is not None else np.zeros(1)
+    amp_a = _f64(amp) if amp is not None else None
+    eref = lib().orc_core_energy(nx, ny, nz, S, _p(np.ascontiguousarray(xi)), int(flag), _p(par),
+                                 int(nm), _p(amp_a), _p(dE), ctypes.byref(ev))
+    return dE, eref, ev.value
+
+
+class GLState:
+    """One reference GL run on a single rank (SolveGL::iterate, solve_GL.cpp:70-158)."""
+
+    def __init__(self, xi, BB, cddt, core_flag, core_par, nm=0, amp=None, xo=None):
+        self.xi = np.ascontiguousarray(xi, dtype=np.complex128).copy()
+        self.S, self.nx, self.ny, self.nz = self.xi.shape
+        self.BB = np.ascontiguousarray(BB, dtype=np.float64)
+        assert self.BB.shape == (self.S, self.S, self.nx, self.ny, self.nz)
+        self.cddt = float(cddt)
+        self.core_flag = int(core_flag)
+        self.core_par = _f64(core_par) if core_par is not None else np.zeros(1)
+        self.nm = int(nm)
+        self.amp = _f64(amp) if amp is not None else None
+        self.xo = _f64(xo) if xo is not None else None
+        self.work = np.zeros(2 * 2 * self.S * self.nx * self.ny * self.nz)
+
+    @property
+    def data_fftw(self):
+        """FFT of the pre-update xi of the last step (what the output step reads)."""
+        n = self.S * self.nx * self.ny * self.nz
+        return self.work[: 2 * n].view(np.complex128).reshape(self.S, self.nx, self.ny, self.nz)
+
+    def step(self, tau) -> dict:
+        out = np.zeros(8)
+        tau = _f64(tau)
+        lib().orc_gl_step(self.nx, self.ny, self.nz, self.S, _p(self.xi), _p(self.xo), _p(self.BB),
+                          self.cddt, _p(tau), self.core_flag, _p(self.core_par), self.nm,
+                          _p(self.amp), _p(self.work), _p(out))
+        keys = ["xinorm", "xiave", "E_el", "E_core", "E_stress", "E_total", "E_core_ref"]
+        return dict(zip(keys, out[:7].tolist()))
+
+
+def diagnostics(data_fftw, xi, FF, eps, mat, mu_s, sigma_rot, fields=False):
+    """Output-step ave_eps / ave_sigma (3x3 each); optional fields."""
+    S, nx, ny, nz = xi.shape
+    ae, asg = np.zeros(9), np.zeros(9)
+    de = np.zeros(2 * 9 * nx * ny * nz) if fields else None
+    ds = np.zeros(2 * 9 * nx * ny * nz) if fields else None
+    lib().orc_diagnostics(nx, ny, nz, S, _p(np.ascontiguousarray(data_fftw)),
+                          _p(np.ascontiguousarray(xi)), _p(_f64(FF)), _p(_f64(eps)),
+                          mat["C44"], mat["C12"], mat["C11"], mat["S12"], mat["S44"], mu_s,
+                          _p(_f64(sigma_rot).reshape(-1)), _p(ae), _p(asg), _p(de), _p(ds))
+    res = {"ave_eps": ae.reshape(3, 3), "ave_sigma": asg.reshape(3, 3)}
+    if fields:
+        res["data_eps"] = de.view(np.complex128).reshape(3, 3, nx, ny, nz)
+        res["data_sigma"] = ds.view(np.complex128).reshape(3, 3, nx, ny, nz)
+    return res
+
+
+class Ranmar:
+    """RanMars (random_mars.cpp:41-101)."""
+
+    def __init__(self, seed: int):
+        self.st = np.zeros(103)
+        lib().orc_ranmar_init(int(seed), _p(self.st))
+
+    def uniform(self) -> float:
+        return lib().orc_ranmar_uniform(_p(self.st))
+
+    def fill(self, n: int) -> np.ndarray:
+        out = np.empty(int(n))
+        lib().orc_ranmar_fill(_p(self.st), int(n), _p(out))
+        return out

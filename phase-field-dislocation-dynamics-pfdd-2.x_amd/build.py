@@ -1,0 +1,40 @@
+"""Compile the gfx950 HIP library lib/libpfdd_gl.so (and the C++ host layer).
+
+hipcc is driven directly (no torch JIT cache), output in-tree so the built
+library ships with the source tree.  All HIP code is built with packed FP32
+instructions disabled.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ARCH = os.environ.get("PFDD_OFFLOAD_ARCH", "gfx950")
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib():
+    os.makedirs(os.path.join(HERE, "lib"), exist_ok=True)
+    out = os.path.join(HERE, "lib", "libpfdd_gl.so")
+    srcs = [os.path.join(HERE, "csrc", "pfdd_gl.hip")]
+    run([HIPCC] + HIPFLAGS + ["-shared", "-o", out + ".tmp"] + srcs)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def main():
+    build_lib()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,280 @@
+// pfdd_fft.hpp -- device building blocks of the batched 3-D FFT used by the GL
+// step: complex arithmetic, radix-2/3/4/8 butterflies, mixed-radix Stockham
+// stages over a tile of lines held in LDS, and coalesced tile load/store.
+//
+// The transform is FFTW 2's (fft_fftw_slab.cpp:1962,2021): unnormalised,
+// forward sign -1, backward sign +1.
+//
+// Tile layout in LDS: element (line l, position p) at lds[p * (NL + 1) + l].
+// Butterflies are mapped line-fastest across threads, so a wave touches
+// consecutive LDS addresses; the +1 pad keeps position-fastest accesses (the
+// coalesced load/store of contiguous lines) free of bank conflicts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfdd {
+
+template <typename R> struct cvec;
+template <> struct cvec<double> { typedef double2 type; };
+template <> struct cvec<float> { typedef float2 type; };
+
+template <typename V> __device__ __forceinline__ V mk(decltype(V{}.x) x, decltype(V{}.x) y)
+{
+    V r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
+// a * w (forward) or a * conj(w) (inverse)
+template <bool INV, typename V> __device__ __forceinline__ V cmulw(V a, V w)
+{
+    const auto wy = INV ? -w.y : w.y;
+    return mk<V>(a.x * w.x - a.y * wy, a.x * wy + a.y * w.x);
+}
+// a * e^{-i pi/2} = -i a (forward), a * e^{+i pi/2} = +i a (inverse)
+template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
+{
+    return INV ? mk<V>(-a.y, a.x) : mk<V>(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- butterflies
+template <bool INV, typename V> __device__ __forceinline__ void dft2(V *v)
+{
+    V a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft4(V *v)
+{
+    V a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    V a2 = cadd(v[1], v[3]), a3 = cmul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft8(V *v)
+{
+    typedef decltype(V{}.x) R;
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<INV>(e);
+    dft4<INV>(o);
+    const R s = (R)0.707106781186547524400844362104849039;
+    V t1, t3;
+    if (INV) {
+        t1 = mk<V>(s * (o[1].x - o[1].y), s * (o[1].x + o[1].y));
+        t3 = mk<V>(-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y));
+    } else {
+        t1 = mk<V>(s * (o[1].x + o[1].y), s * (o[1].y - o[1].x));
+        t3 = mk<V>(s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y));
+    }
+    V t2 = cmul_mi<INV>(o[2]);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft3(V *v)
+{
+    typedef decltype(V{}.x) R;
+    const R h = (R)0.866025403784438646763723170752936183; // sqrt(3)/2
+    V t = cadd(v[1], v[2]);
+    V d = csub(v[1], v[2]);
+    V y0 = cadd(v[0], t);
+    V m = mk<V>(v[0].x - (R)0.5 * t.x, v[0].y - (R)0.5 * t.y);
+    // forward: y1 = m - i h d, y2 = m + i h d ; inverse swaps
+    V ihd = mk<V>(-h * d.y, h * d.x); // +i h d
+    v[0] = y0;
+    if (INV) {
+        v[1] = cadd(m, ihd);
+        v[2] = csub(m, ihd);
+    } else {
+        v[1] = csub(m, ihd);
+        v[2] = cadd(m, ihd);
+    }
+}
+
+template <int RAD, bool INV, typename V> __device__ __forceinline__ void dft(V *v)
+{
+    if constexpr (RAD == 2) dft2<INV>(v);
+    else if constexpr (RAD == 3) dft3<INV>(v);
+    else if constexpr (RAD == 4) dft4<INV>(v);
+    else if constexpr (RAD == 8) dft8<INV>(v);
+    else static_assert(RAD == 2, "unsupported radix");
+}
+
+constexpr int pick_radix(int m)
+{
+    return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m;
+}
+
+// Lines per LDS tile for a transform of length N: a power of two giving about
+// 32 KiB of complex data per tile.
+template <typename R, int N> constexpr int tile_lines()
+{
+    constexpr int target = sizeof(R) == 8 ? 2048 : 4096;
+    int nl = 1;
+    while (nl * 2 * N <= target) nl *= 2;
+    return nl;
+}
+
+// One Stockham (auto-sort) radix-RAD stage over all NL lines of the tile,
+// in place in LDS: every thread reads all its inputs, barrier, writes.
+//   input  (line l, j + r*N/RAD), r < RAD, for butterfly j < N/RAD
+//   twiddle w^{r*(j mod NS)}, w = exp(-+2 pi i / (NS*RAD))
+//   output (line l, (j/NS)*NS*RAD + (j mod NS) + r*NS)
+// tw[m] = exp(-2 pi i m / N), m < N.
+template <int N, int RAD, int NS, int NL, int NTHR, bool INV, typename V>
+__device__ __forceinline__ void stockham_stage(V *lds, const V *__restrict__ tw, int tid)
+{
+    constexpr int LDP = NL + 1;
+    constexpr int NBL = N / RAD;
+    constexpr int NBF = NBL * NL;
+    constexpr int PER = (NBF + NTHR - 1) / NTHR;
+    V v[PER][RAD];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int bf = tid + q * NTHR;
+        if ((NBF % NTHR == 0) || bf < NBF) {
+            const int l = bf % NL, j = bf / NL;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[q][r] = lds[(j + r * NBL) * LDP + l];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int bf = tid + q * NTHR;
+        if ((NBF % NTHR == 0) || bf < NBF) {
+            const int l = bf % NL, j = bf / NL;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r)
+                    v[q][r] = cmulw<INV>(v[q][r], tw[r * k * (N / (NS * RAD))]);
+            }
+            dft<RAD, INV>(v[q]);
+            const int idxD = (j / NS) * NS * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) lds[(idxD + r * NS) * LDP + l] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NS, int NL, int NTHR, bool INV, typename V>
+__device__ __forceinline__ void fft_stages(V *lds, const V *__restrict__ tw, int tid)
+{
+    if constexpr (NS < N) {
+        constexpr int RAD = pick_radix(N / NS);
+        static_assert(RAD == 2 || RAD == 3 || RAD == 4 || RAD == 8, "length must be 2^a 3^b");
+        stockham_stage<N, RAD, NS, NL, NTHR, INV, V>(lds, tw, tid);
+        fft_stages<N, NS * RAD, NL, NTHR, INV, V>(lds, tw, tid);
+    }
+}
+
+// Addressing of one line of a pass: line index L is split as (oa, ob, d),
+// d fastest (d < D, ob < B); element p of the line sits at
+//   oa*SA + ob*SB + d*SD + p*SP.
+struct LineMap {
+    long long SA, SB, SD, SP;
+    long long D, B;
+};
+
+__device__ __forceinline__ long long line_base(const LineMap &m, long long L)
+{
+    const long long d = L % m.D, t = L / m.D;
+    const long long ob = t % m.B, oa = t / m.B;
+    return oa * m.SA + ob * m.SB + d * m.SD;
+}
+
+// Global -> LDS tile.  base[l] < 0 marks an inactive line.  Position-fastest
+// when the line is contiguous in memory (SP == 1), line-fastest otherwise
+// (then consecutive lines are contiguous: SD == 1).
+template <int N, int NL, int NTHR, typename V>
+__device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
+                                          const long long *base, long long SP, int tid)
+{
+    constexpr int LDP = NL + 1, TOT = N * NL;
+    if (SP == 1) {
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e / N, p = e % N;
+            const long long b = base[l];
+            if (b >= 0) lds[p * LDP + l] = src[b + p];
+        }
+    } else {
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e % NL, p = e / NL;
+            const long long b = base[l];
+            if (b >= 0) lds[p * LDP + l] = src[b + p * SP];
+        }
+    }
+}
+
+template <int N, int NL, int NTHR, typename V>
+__device__ __forceinline__ void tile_store(V *__restrict__ dst, const V *lds,
+                                           const long long *base, long long SP, int tid)
+{
+    constexpr int LDP = NL + 1, TOT = N * NL;
+    if (SP == 1) {
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e / N, p = e % N;
+            const long long b = base[l];
+            if (b >= 0) dst[b + p] = lds[p * LDP + l];
+        }
+    } else {
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e % NL, p = e / NL;
+            const long long b = base[l];
+            if (b >= 0) dst[b + p * SP] = lds[p * LDP + l];
+        }
+    }
+}
+
+// ------------------------------------------------------------- reductions
+// Block-wide sum of NV doubles (NTHR threads, wave64).  Result valid in
+// thread 0.  Uses `red` (NV * NTHR/64 doubles of LDS).
+template <int NV, int NTHR>
+__device__ __forceinline__ void block_sum(double *v, double *red, int tid)
+{
+    constexpr int NW = NTHR / 64;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[q] = x;
+    }
+    const int lane = tid & 63, w = tid >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) red[q * NW + w] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int i = 0; i < NW; ++i) s += red[q * NW + i];
+            v[q] = s;
+        }
+    }
+}
+
+} // namespace pfdd

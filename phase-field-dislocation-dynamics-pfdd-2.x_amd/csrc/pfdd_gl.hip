@@ -1,0 +1,1222 @@
+// pfdd_gl.hip -- gfx950 implementation of the PFDD Ginzburg-Landau step and
+// its C ABI (include/pfdd_gl.h).
+//
+// One GL iteration of SolveGL::iterate (solve_GL.cpp:86-148) is five kernels
+// over the S order parameters of an nx*ny*nz grid, with the field kept in
+// three device buffers (two ping-pong copies of xi and one temporary T1):
+//
+//   1  fwd z   xi[a][i][j][k]  -> T1[a][i][k][j]   FFT along k, transposed write
+//   2  fwd y   T1[a][i][k][j]  -> T2[a][k][j][i]   FFT along j, transposed write
+//   3  x+B     T2 in place: FFT along i, ghat_a = sum_b B_ab xihat_b (packed
+//              symmetric B, FP64 or FP32), inverse FFT along i, plus the
+//              elastic-energy partial sums
+//   4  inv y   T2 -> T1[a][i][k][j]
+//   5  inv z   T1 -> g, fused core-energy derivative, applied stress, explicit
+//              Euler update, obstacle mask and reductions: xi_old -> xi_new
+//   +  one single-block kernel reduces the per-block partials in fixed order.
+//
+// T2 is the ping-pong copy of xi that is not current, so the whole iteration
+// needs 3*S*N complex values plus B.  Every global access is a run of at least
+// 8 consecutive complex values (the pass transposes through LDS), and every
+// kernel reads and writes each field element exactly once.
+//
+// Reference functions replaced (fft_fftw_slab.cpp): core_energy_* :1130-1418,
+// forward_mode1 :1953-1964, internal_energy :1670-1695, backward_mode1
+// :2012-2023, update_order_parameter :1746-1790, prepare_next_itr :1838-1862,
+// init_loop :1628-1666 (no longer needed), Bmatrix :466-638, frec :280-367.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pfdd_fft.hpp"
+#include "../../include/pfdd_gl.h"
+
+using namespace pfdd;
+
+namespace {
+
+constexpr int MAXS = PFDD_GL_MAX_SYSTEMS;
+constexpr int MAXM = PFDD_GL_MAX_COEF;
+constexpr int NTHR = 256;
+constexpr int NSLOT = 5; // xinorm^2, xiave, e_core, e_stress, e_el
+constexpr int NRES = 8;  // xinorm, xiave, e_el, e_core, e_stress, e_total, e_core_ref, nonfinite
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(PFDD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+struct StepParams {
+    double tau[MAXS];
+    double coef[MAXS * MAXM];
+    double cddt;
+    double invN;
+    int ncoef;
+    int S;
+};
+
+struct BParams {
+    double sig[MAXS * 9]; // sigma_a = C : eps_a
+    double eps[MAXS * 9]; // eps_a = b_a n_a^T / dslip
+    double four[9];
+    double inv2C44;       // 1 / (2 C44)
+    double inv1mnu;       // 1 / (1 - xnu)
+    double invmu;         // 1 / mu (Voigt mu of Bmatrix)
+    int nx, ny, nz, S;
+};
+
+__device__ __forceinline__ double frac_freq(int kk, int n)
+{
+    // frec(), fft_fftw_slab.cpp:294-322
+    return kk >= n / 2 ? (double)(kk - n) / (double)n : (double)kk / (double)n;
+}
+
+// ---------------------------------------------------------------- B(k) build
+// Packed symmetric B in the point order of T2 (index (k*ny + j)*nx + i), pair
+// (a<=b) at slot a*S - a(a-1)/2 + (b-a).  Closed form of Bmatrix's A-tensor
+// contraction (fft_fftw_slab.cpp:577-634):
+//   mu B_ab = sig_a : eps_b - [2 (u_a . u_b)/f^2 - (u_a.f)(u_b.f)/((1-nu) f^4)] / (2 C44)
+// with u_a = sig_a f; B(0) = 0.
+template <typename RB>
+__global__ void __launch_bounds__(NTHR) k_bbuild(RB *__restrict__ B, BParams P, long long ntot)
+{
+    const long long q = (long long)blockIdx.x * NTHR + threadIdx.x;
+    if (q >= ntot) return;
+    const int i = (int)(q % P.nx);
+    const long long col = q / P.nx;
+    const int j = (int)(col % P.ny);
+    const int k = (int)(col / P.ny);
+    const double kx = frac_freq(i, P.nx), ky = frac_freq(j, P.ny), kz = frac_freq(k, P.nz);
+    double f[3];
+    for (int c = 0; c < 3; ++c) f[c] = kx * P.four[0 + c] + ky * P.four[3 + c] + kz * P.four[6 + c];
+    const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    const int S = P.S;
+    double u[MAXS][3], uf[MAXS];
+    for (int a = 0; a < S; ++a) {
+        for (int r = 0; r < 3; ++r)
+            u[a][r] = P.sig[a * 9 + r * 3 + 0] * f[0] + P.sig[a * 9 + r * 3 + 1] * f[1] +
+                      P.sig[a * 9 + r * 3 + 2] * f[2];
+        uf[a] = u[a][0] * f[0] + u[a][1] * f[1] + u[a][2] * f[2];
+    }
+    const double if2 = f2 > 0 ? 1.0 / f2 : 0.0;
+    const double if4 = if2 * if2;
+    int slot = 0;
+    for (int a = 0; a < S; ++a)
+        for (int b = a; b < S; ++b, ++slot) {
+            double v = 0.0;
+            if (f2 > 0) {
+                double ce = 0.0;
+                for (int m = 0; m < 9; ++m) ce += P.sig[a * 9 + m] * P.eps[b * 9 + m];
+                const double uu = u[a][0] * u[b][0] + u[a][1] * u[b][1] + u[a][2] * u[b][2];
+                v = (ce - (2.0 * uu * if2 - P.inv1mnu * uf[a] * uf[b] * if4) * P.inv2C44) * P.invmu;
+            }
+            B[(long long)slot * ntot + q] = (RB)v;
+        }
+}
+
+// ------------------------------------------------- generic FFT pass (1, 2, 4)
+template <typename R, int N, bool INV>
+__global__ void __launch_bounds__(NTHR)
+    k_fft_pass(const typename cvec<R>::type *__restrict__ in, typename cvec<R>::type *__restrict__ out,
+               LineMap mi, LineMap mo, long long nlines, const typename cvec<R>::type *__restrict__ tw,
+               const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1;
+    __shared__ V lds[N * LDP];
+    __shared__ long long bin[NL], bout[NL];
+    const int tid = threadIdx.x;
+    const long long L0 = (long long)blockIdx.x * NL;
+    for (int l = tid; l < NL; l += NTHR) {
+        const long long L = L0 + l;
+        const bool ok = L < nlines;
+        bin[l] = ok ? line_base(mi, L) : -1;
+        bout[l] = ok ? line_base(mo, L) : -1;
+    }
+    __syncthreads();
+    tile_load<N, NL, NTHR>(lds, in, bin, mi.SP, tid);
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
+    tile_store<N, NL, NTHR>(out, lds, bout, mo.SP, tid);
+}
+
+// --------------------------------------------- pass 3: x FFT + contraction
+template <typename R, int N, int S> constexpr int mid_pairs()
+{
+    constexpr int target = sizeof(R) == 8 ? 2048 : 4096;
+    int np = 1;
+    while (np * 2 * S * N <= target) np *= 2;
+    return np;
+}
+
+template <typename R, typename RB, int N, int S>
+__global__ void __launch_bounds__(NTHR)
+    k_mid(typename cvec<R>::type *__restrict__ T2, const RB *__restrict__ B, long long ncol,
+          long long ntot, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
+          const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NP = mid_pairs<R, N, S>();
+    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
+    constexpr int NPAIR = S * (S + 1) / 2;
+    __shared__ V lds[N * LDP];
+    __shared__ double red[NTHR / 64];
+    const int tid = threadIdx.x;
+    const long long c0 = (long long)blockIdx.x * NP;
+    // load: lines (a, q) -> T2[a][c0+q][0..N)
+#pragma unroll 4
+    for (int e = tid; e < TOT; e += NTHR) {
+        const int l = e / N, p = e % N;
+        const int a = l / NP, q = l % NP;
+        if (c0 + q < ncol) lds[p * LDP + l] = T2[(long long)a * ntot + (c0 + q) * N + p];
+    }
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
+    double el = 0.0;
+    for (int e = tid; e < NP * N; e += NTHR) {
+        const int q = e / N, p = e % N;
+        if (c0 + q >= ncol) continue;
+        const long long kp = (c0 + q) * N + p;
+        V x[S];
+#pragma unroll
+        for (int b = 0; b < S; ++b) x[b] = lds[p * LDP + b * NP + q];
+        R g[S][2];
+#pragma unroll
+        for (int a = 0; a < S; ++a) g[a][0] = g[a][1] = (R)0;
+        int slot = 0;
+#pragma unroll
+        for (int a = 0; a < S; ++a) {
+#pragma unroll
+            for (int b = a; b < S; ++b, ++slot) {
+                const R bab = (R)B[(long long)slot * ntot + kp];
+                g[a][0] += bab * x[b].x;
+                g[a][1] += bab * x[b].y;
+                if (b != a) {
+                    g[b][0] += bab * x[a].x;
+                    g[b][1] += bab * x[a].y;
+                }
+            }
+        }
+        (void)NPAIR;
+#pragma unroll
+        for (int a = 0; a < S; ++a) {
+            el += (double)x[a].x * (double)g[a][0] + (double)x[a].y * (double)g[a][1];
+            lds[p * LDP + a * NP + q] = mk<V>(g[a][0], g[a][1]);
+        }
+    }
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+#pragma unroll 4
+    for (int e = tid; e < TOT; e += NTHR) {
+        const int l = e / N, p = e % N;
+        const int a = l / NP, q = l % NP;
+        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds[p * LDP + l];
+    }
+    double v[1] = {el};
+    block_sum<1, NTHR>(v, red, tid);
+    if (tid == 0) part_el[blockIdx.x] = v[0];
+}
+
+// ------------------------------------- pass 5: inverse z FFT + fused update
+// update_order_parameter (fft_fftw_slab.cpp:1746-1790) with the core-energy
+// derivative of core_energy_* evaluated on the old field, as the reference
+// does (it reads data_fftw = xi before the update).  Masked cells keep their
+// value; their xinorm term uses the previous unmasked cell's old value in
+// (a, i, j, k) order (prev[] precomputed on the host, -1 = the initial 0).
+template <typename R, typename RA, int N>
+__global__ void __launch_bounds__(NTHR)
+    k_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ xo,
+             typename cvec<R>::type *__restrict__ xn, const uint8_t *__restrict__ mask,
+             const long long *__restrict__ prev, const RA *__restrict__ amp, LineMap mi, LineMap mo,
+             long long nlines, long long lines_per_sys, StepParams P,
+             const typename cvec<R>::type *__restrict__ tw, double *__restrict__ partial,
+             long long pstride, const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
+    __shared__ V lds[N * LDP];
+    __shared__ long long bin[NL], bout[NL];
+    __shared__ int lsys[NL];
+    __shared__ double red[4 * (NTHR / 64)];
+    const int tid = threadIdx.x;
+    const long long L0 = (long long)blockIdx.x * NL;
+    for (int l = tid; l < NL; l += NTHR) {
+        const long long L = L0 + l;
+        const bool ok = L < nlines;
+        bin[l] = ok ? line_base(mi, L) : -1;
+        bout[l] = ok ? line_base(mo, L) : -1;
+        lsys[l] = ok ? (int)(L / lines_per_sys) : 0;
+    }
+    __syncthreads();
+    tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+
+    const R cddt = (R)P.cddt, invN = (R)P.invN;
+    const int nm = P.ncoef;
+    double s_norm = 0.0, s_ave = 0.0, s_core = 0.0, s_str = 0.0;
+#pragma unroll 2
+    for (int e = tid; e < TOT; e += NTHR) {
+        const int l = e / N, p = e % N; // mo.SP == 1: rows of xi are contiguous
+        const long long b = bout[l];
+        if (b < 0) continue;
+        const long long idx = b + p;
+        const int a = lsys[l];
+        const V g = lds[p * LDP + l];
+        const V x = xo[idx];
+        // core energy density and its derivative at Re xi (sine series)
+        R sp, cp;
+        if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
+        else sincospif((float)x.x, (float *)&sp, (float *)&cp);
+        R sm = sp, cm = cp, dens = (R)0, dE = (R)0;
+        for (int m = 1; m <= nm; ++m) {
+            const R c = (R)P.coef[a * MAXM + m - 1];
+            dens += c * sm * sm;
+            dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
+            const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
+            sm = sn;
+            cm = cn;
+        }
+        R A = (R)1;
+        if (amp) A = (R)amp[idx];
+        dens *= A;
+        dE *= A;
+        const R tau = (R)P.tau[a];
+        V y;
+        double dn;
+        if (mask && mask[idx]) {
+            y = x;
+            V pv = mk<V>((R)0, (R)0);
+            const long long pi = prev[idx];
+            if (pi >= 0) pv = xo[pi];
+            const double dr = (double)x.x - (double)pv.x, di = (double)x.y - (double)pv.y;
+            dn = dr * dr + di * di;
+        } else {
+            y.x = x.x - cddt * (g.x * invN - tau + dE);
+            y.y = x.y - cddt * (g.y * invN);
+            const double dr = (double)y.x - (double)x.x, di = (double)y.y - (double)x.y;
+            dn = dr * dr + di * di;
+        }
+        xn[idx] = y;
+        s_norm += dn;
+        s_ave += (double)y.x;
+        s_core += (double)dens;
+        s_str += (double)tau * (double)x.x;
+    }
+    double v[4] = {s_norm, s_ave, s_core, s_str};
+    block_sum<4, NTHR>(v, red, tid);
+    if (tid == 0) {
+        partial[0 * pstride + blockIdx.x] = v[0];
+        partial[1 * pstride + blockIdx.x] = v[1];
+        partial[2 * pstride + blockIdx.x] = v[2];
+        partial[3 * pstride + blockIdx.x] = v[3];
+    }
+}
+
+// Deterministic final reduction of the block partials (Kahan within each
+// thread's strided sweep, then a fixed-shape tree).  Writes the iteration's
+// scalars, counts the iteration and raises `done` on convergence or on a
+// non-finite result.
+__global__ void __launch_bounds__(NTHR)
+    k_finalize(const double *__restrict__ partial, long long pstride, int nb5, int nb3, double invN,
+               double inv_nx, double tol, double *__restrict__ res, int *__restrict__ done,
+               int *__restrict__ steps)
+{
+    if (*done) return;
+    __shared__ double red[NSLOT * (NTHR / 64)];
+    const int tid = threadIdx.x;
+    double v[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        const int nb = s == 4 ? nb3 : nb5;
+        double sum = 0.0, c = 0.0;
+        for (int b = tid; b < nb; b += NTHR) {
+            const double y = partial[s * pstride + b] - c;
+            const double t = sum + y;
+            c = (t - sum) - y;
+            sum = t;
+        }
+        v[s] = sum;
+    }
+    block_sum<NSLOT, NTHR>(v, red, tid);
+    if (tid == 0) {
+        const double xinorm = sqrt(v[0]);
+        const double e_el = 0.5 * v[4] * invN;
+        res[0] = xinorm;
+        res[1] = v[1];
+        res[2] = e_el;
+        res[3] = v[2];
+        res[4] = v[3];
+        res[5] = e_el + v[2] - v[3];
+        res[6] = v[2] * inv_nx;
+        bool finite = isfinite(xinorm) && isfinite(v[1]) && isfinite(e_el) && isfinite(v[2]);
+        res[7] = finite ? 0.0 : 1.0;
+        *steps += 1;
+        if (!finite || (tol > 0 && xinorm < tol)) *done = finite ? 1 : 2;
+    }
+}
+
+__global__ void k_reset(int *done, int *steps)
+{
+    if (threadIdx.x == 0) {
+        *done = 0;
+        *steps = 0;
+    }
+}
+
+// Per-system sums of Re xi: grid (nb, S); one partial per block.
+template <typename R>
+__global__ void __launch_bounds__(NTHR)
+    k_sys_sums(const typename cvec<R>::type *__restrict__ xi, long long ntot, double *__restrict__ part)
+{
+    __shared__ double red[NTHR / 64];
+    const int a = blockIdx.y;
+    const long long chunk = (ntot + gridDim.x - 1) / gridDim.x;
+    const long long lo = (long long)blockIdx.x * chunk;
+    const long long hi = lo + chunk < ntot ? lo + chunk : ntot;
+    double s = 0.0;
+    for (long long r = lo + threadIdx.x; r < hi; r += NTHR) s += (double)xi[(long long)a * ntot + r].x;
+    double v[1] = {s};
+    block_sum<1, NTHR>(v, red, threadIdx.x);
+    if (threadIdx.x == 0) part[(long long)a * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename R>
+__global__ void k_from_f64(const double2 *__restrict__ src, typename cvec<R>::type *__restrict__ dst, long long n)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = mk<typename cvec<R>::type>((R)src[q].x, (R)src[q].y);
+}
+template <typename R>
+__global__ void k_to_f64(const typename cvec<R>::type *__restrict__ src, double2 *__restrict__ dst, long long n)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = make_double2((double)src[q].x, (double)src[q].y);
+}
+
+// ------------------------------------------------------------------ dispatch
+#define PFDD_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
+
+bool size_supported(int n)
+{
+#define PFDD_CASE(n_) if (n == n_) return true;
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return false;
+}
+
+bool systems_supported(int s) { return s == 1 || s == 2 || s == 3 || s == 4 || s == 6 || s == 12; }
+
+template <typename R>
+int tile_lines_rt(int n)
+{
+#define PFDD_CASE(n_) if (n == n_) return tile_lines<R, n_>();
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return 0;
+}
+
+template <typename R, bool INV>
+hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo, long long nlines,
+                       const void *tw, const int *done, hipStream_t st)
+{
+    typedef typename cvec<R>::type V;
+#define PFDD_CASE(n_)                                                                           \
+    if (n == n_) {                                                                              \
+        constexpr int NL = tile_lines<R, n_>();                                                 \
+        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
+        hipLaunchKernelGGL((k_fft_pass<R, n_, INV>), dim3(nb), dim3(NTHR), 0, st, (const V *)in, \
+                           (V *)out, mi, mo, nlines, (const V *)tw, done);                      \
+        return hipGetLastError();                                                               \
+    }
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R, typename RA>
+hipError_t launch_update(int n, const void *T1, const void *xo, void *xn, const uint8_t *mask,
+                         const long long *prev, const void *amp, LineMap mi, LineMap mo, long long nlines,
+                         long long lps, const StepParams &P, const void *tw, double *partial,
+                         long long pstride, const int *done, hipStream_t st)
+{
+    typedef typename cvec<R>::type V;
+#define PFDD_CASE(n_)                                                                            \
+    if (n == n_) {                                                                               \
+        constexpr int NL = tile_lines<R, n_>();                                                  \
+        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
+        hipLaunchKernelGGL((k_update<R, RA, n_>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,    \
+                           (const V *)xo, (V *)xn, mask, prev, (const RA *)amp, mi, mo, nlines, lps, \
+                           P, (const V *)tw, partial, pstride, done);                            \
+        return hipGetLastError();                                                                \
+    }
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R, int S>
+int mid_blocks_rt(int n, long long ncol)
+{
+#define PFDD_CASE(n_) if (n == n_) return (int)((ncol + mid_pairs<R, n_, S>() - 1) / mid_pairs<R, n_, S>());
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return 0;
+}
+
+template <typename R>
+int mid_blocks(int n, int S, long long ncol)
+{
+    switch (S) {
+    case 1: return mid_blocks_rt<R, 1>(n, ncol);
+    case 2: return mid_blocks_rt<R, 2>(n, ncol);
+    case 3: return mid_blocks_rt<R, 3>(n, ncol);
+    case 4: return mid_blocks_rt<R, 4>(n, ncol);
+    case 6: return mid_blocks_rt<R, 6>(n, ncol);
+    case 12: return mid_blocks_rt<R, 12>(n, ncol);
+    }
+    return 0;
+}
+
+template <typename R, int N, int S> constexpr bool mid_fits()
+{
+    return (long long)N * (S * mid_pairs<R, N, S>() + 1) * 2 * sizeof(R) <= 65536;
+}
+
+template <typename R, int S>
+hipError_t launch_mid_s(int n, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
+                        double *part_el, const int *done, hipStream_t st)
+{
+    typedef typename cvec<R>::type V;
+#define PFDD_CASE(n_)                                                                             \
+    if (n == n_) {                                                                                \
+        if constexpr (mid_fits<R, n_, S>()) {                                                     \
+            constexpr int NP = mid_pairs<R, n_, S>();                                             \
+            const unsigned nb = (unsigned)((ncol + NP - 1) / NP);                                 \
+            hipLaunchKernelGGL((k_mid<R, R, n_, S>), dim3(nb), dim3(NTHR), 0, st, (V *)T2,        \
+                               (const R *)B, ncol, ntot, (const V *)tw, part_el, done);           \
+            return hipGetLastError();                                                             \
+        } else {                                                                                  \
+            return hipErrorInvalidValue;                                                          \
+        }                                                                                         \
+    }
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+bool mid_supported(int n, int S)
+{
+    auto chk = [&](auto sconst) -> bool {
+        constexpr int SS = decltype(sconst)::value;
+#define PFDD_CASE(n_) if (n == n_) return mid_fits<R, n_, SS>();
+        PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+        return false;
+    };
+    switch (S) {
+    case 1: return chk(std::integral_constant<int, 1>());
+    case 2: return chk(std::integral_constant<int, 2>());
+    case 3: return chk(std::integral_constant<int, 3>());
+    case 4: return chk(std::integral_constant<int, 4>());
+    case 6: return chk(std::integral_constant<int, 6>());
+    case 12: return chk(std::integral_constant<int, 12>());
+    }
+    return false;
+}
+
+template <typename R>
+hipError_t launch_mid(int n, int S, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
+                      double *part_el, const int *done, hipStream_t st)
+{
+    switch (S) {
+    case 1: return launch_mid_s<R, 1>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 2: return launch_mid_s<R, 2>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 3: return launch_mid_s<R, 3>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 4: return launch_mid_s<R, 4>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 6: return launch_mid_s<R, 6>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 12: return launch_mid_s<R, 12>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+
+// ================================================================== handle
+struct pfdd_gl {
+    int nx = 0, ny = 0, nz = 0, S = 0, prec = PFDD_FP64;
+    long long N = 0;
+    size_t esz = 16; // bytes per complex element
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void *xi[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *T1 = nullptr;
+    void *B = nullptr;
+    void *tw[3] = {nullptr, nullptr, nullptr}; // x, y, z twiddle tables
+    uint8_t *mask = nullptr;
+    long long *prev = nullptr;
+    void *amp = nullptr;
+    double *partial = nullptr;
+    long long pstride = 0;
+    int nb3 = 0, nb5 = 0;
+    double *res = nullptr; // NRES doubles
+    int *done = nullptr;
+    int *steps = nullptr;
+    StepParams P{};
+    // host copies for diagnostics
+    double xn[MAXS * 3], xbv[MAXS * 3], eps[MAXS * 9];
+    double C44 = 0, C12 = 0, C11 = 0, S12 = 0, S44 = 0, dslip = 0, mu = 0, xnu = 0;
+    double four[9];
+    bool stepped = false; // a step ran since the last set_xi (xi[1-cur] holds the pre-update field)
+};
+
+namespace {
+
+void make_twiddles(int n, std::vector<double> &tw)
+{
+    tw.resize(2 * n);
+    for (int m = 0; m < n; ++m) {
+        // exp(-2 pi i m / n), evaluated with the angle folded to the first octant
+        long long mm = m;
+        double c, s;
+        const long long n8 = 8LL * mm;
+        // use sincospi-like exactness at multiples of n/8
+        if (n8 % n == 0) {
+            const int oct = (int)(n8 / n);
+            static const double r2 = 0.70710678118654752440;
+            const double cs[8] = {1, r2, 0, -r2, -1, -r2, 0, r2};
+            const double sn[8] = {0, r2, 1, r2, 0, -r2, -1, -r2};
+            c = cs[oct];
+            s = sn[oct];
+        } else {
+            const double ang = 2.0 * M_PI * (double)mm / (double)n;
+            c = std::cos(ang);
+            s = std::sin(ang);
+        }
+        tw[2 * m] = c;
+        tw[2 * m + 1] = -s;
+    }
+}
+
+int upload_twiddles(pfdd_gl *h)
+{
+    const int dims[3] = {h->nx, h->ny, h->nz};
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double> tw;
+        make_twiddles(dims[d], tw);
+        const size_t bytes = (size_t)dims[d] * h->esz;
+        HIPCHK(hipMalloc(&h->tw[d], bytes));
+        if (h->prec == PFDD_FP64) {
+            HIPCHK(hipMemcpy(h->tw[d], tw.data(), bytes, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(tw.begin(), tw.end());
+            HIPCHK(hipMemcpy(h->tw[d], f.data(), bytes, hipMemcpyHostToDevice));
+        }
+    }
+    return PFDD_OK;
+}
+
+// Line maps of the passes (see the file header).
+struct Maps {
+    LineMap p1i, p1o, p2i, p2o, p5i, p5o;
+    long long n1, n2, n5;
+};
+
+Maps make_maps(const pfdd_gl *h)
+{
+    const long long nx = h->nx, ny = h->ny, nz = h->nz, S = h->S;
+    Maps m;
+    // pass 1: lines (a,i | j), FFT along k
+    m.p1i = {ny * nz, 0, nz, 1, ny, 1};
+    m.p1o = {nz * ny, 0, 1, ny, ny, 1};
+    m.n1 = S * nx * ny;
+    // pass 2 / 4: lines (a | k | i), FFT along j; T1[a][i][k][j] <-> T2[a][k][j][i]
+    m.p2i = {nx * nz * ny, ny, nz * ny, 1, nx, nz};
+    m.p2o = {nz * ny * nx, ny * nx, 1, nx, nx, nz};
+    m.n2 = S * nz * nx;
+    // pass 5: lines (a,i | j), FFT along k; T1[a][i][k][j] -> xi[a][i][j][k]
+    m.p5i = {nz * ny, 0, 1, ny, ny, 1};
+    m.p5o = {ny * nz, 0, nz, 1, ny, 1};
+    m.n5 = S * nx * ny;
+    return m;
+}
+
+template <typename R>
+int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
+{
+    const Maps m = make_maps(h);
+    void *xo = h->xi[h->cur];
+    void *T2 = h->xi[1 - h->cur];
+    hipStream_t st = h->stream;
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st)));
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK((launch_mid<R>(h->nx, h->S, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
+                          h->partial + 4 * h->pstride, h->done, st)));
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
+    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+    hipError_t e;
+    if (h->prec == PFDD_FP64)
+        e = launch_update<R, double>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
+                                     (long long)h->nx * h->ny, h->P, h->tw[2], h->partial, h->pstride,
+                                     h->done, st);
+    else
+        e = launch_update<R, float>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
+                                    (long long)h->nx * h->ny, h->P, h->tw[2], h->partial, h->pstride,
+                                    h->done, st);
+    HIPCHK(e);
+    if (ev) HIPCHK(hipEventRecord(ev[5], st));
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTHR), 0, st, h->partial, h->pstride, h->nb5, h->nb3,
+                       1.0 / (double)h->N, 1.0 / (double)h->nx, tol, h->res, h->done, h->steps);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[6], st));
+    h->cur = 1 - h->cur;
+    h->stepped = true;
+    return PFDD_OK;
+}
+
+int check_handle(const pfdd_gl *h)
+{
+    if (!h) return fail(PFDD_ERR_ARG, "null handle");
+    return PFDD_OK;
+}
+
+int set_device(const pfdd_gl *h)
+{
+    HIPCHK(hipSetDevice(h->device));
+    return PFDD_OK;
+}
+
+void destroy_all(pfdd_gl *h)
+{
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
+                    h->amp, h->partial, h->res, h->done, h->steps};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int set_coef(pfdd_gl *h, int model, int ncoef, const double *coef, bool has_amp)
+{
+    StepParams &P = h->P;
+    for (int q = 0; q < MAXS * MAXM; ++q) P.coef[q] = 0.0;
+    switch (model) {
+    case PFDD_CORE_NONE:
+        P.ncoef = 0;
+        break;
+    case PFDD_CORE_PERFECT:
+        if (!coef) return fail(PFDD_ERR_ARG, "core model 1 needs coef[0] = An'");
+        P.ncoef = 1;
+        for (int a = 0; a < h->S; ++a) P.coef[a * MAXM] = coef[0];
+        break;
+    case PFDD_CORE_SINE:
+        if (!coef) return fail(PFDD_ERR_ARG, "core model 3 needs coef[0..1] = isf', Cn'");
+        P.ncoef = 2;
+        for (int a = 0; a < h->S; ++a) {
+            P.coef[a * MAXM] = coef[0];
+            P.coef[a * MAXM + 1] = coef[1];
+        }
+        break;
+    case PFDD_CORE_MPEA:
+        if (!has_amp) return fail(PFDD_ERR_ARG, "core model 7 needs the per-cell amplitude amp[]");
+        P.ncoef = 1;
+        for (int a = 0; a < h->S; ++a) P.coef[a * MAXM] = 1.0;
+        break;
+    case PFDD_CORE_SERIES:
+        if (ncoef < 0 || ncoef > MAXM) return fail(PFDD_ERR_ARG, "ncoef must be in [0, %d]", MAXM);
+        if (ncoef > 0 && !coef) return fail(PFDD_ERR_ARG, "core model 8 needs coef[ns*ncoef]");
+        P.ncoef = ncoef;
+        for (int a = 0; a < h->S; ++a)
+            for (int m = 0; m < ncoef; ++m) P.coef[a * MAXM + m] = coef[a * ncoef + m];
+        break;
+    default:
+        return fail(PFDD_ERR_ARG, "unknown core model %d", model);
+    }
+    return PFDD_OK;
+}
+
+template <typename R>
+int build_B(pfdd_gl *h)
+{
+    BParams bp{};
+    const int S = h->S;
+    for (int a = 0; a < S; ++a) {
+        double e[9], s[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) e[3 * i + j] = h->xbv[3 * a + i] * h->xn[3 * a + j] / h->dslip;
+        const double tr = e[0] + e[4] + e[8];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                s[3 * i + j] = h->C44 * (e[3 * i + j] + e[3 * j + i]) + (i == j ? h->C12 * tr : 0.0);
+        for (int q = 0; q < 9; ++q) {
+            bp.eps[a * 9 + q] = e[q];
+            bp.sig[a * 9 + q] = s[q];
+            h->eps[a * 9 + q] = e[q];
+        }
+    }
+    for (int q = 0; q < 9; ++q) bp.four[q] = h->four[q];
+    bp.inv2C44 = 1.0 / (2.0 * h->C44);
+    bp.inv1mnu = 1.0 / (1.0 - h->xnu);
+    bp.invmu = 1.0 / h->mu;
+    bp.nx = h->nx;
+    bp.ny = h->ny;
+    bp.nz = h->nz;
+    bp.S = S;
+    const long long np = (long long)S * (S + 1) / 2;
+    HIPCHK(hipMalloc(&h->B, (size_t)np * h->N * sizeof(R)));
+    const unsigned nb = (unsigned)((h->N + NTHR - 1) / NTHR);
+    hipLaunchKernelGGL((k_bbuild<R>), dim3(nb), dim3(NTHR), 0, h->stream, (R *)h->B, bp, h->N);
+    HIPCHK(hipGetLastError());
+    return PFDD_OK;
+}
+
+int upload_field_f64(pfdd_gl *h, const double *host, void *dst)
+{
+    const long long n = (long long)h->S * h->N;
+    if (h->prec == PFDD_FP64) {
+        HIPCHK(hipMemcpyAsync(dst, host, (size_t)n * 16, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PFDD_OK;
+    }
+    // FP32: stage through T1 as FP64 would not fit (T1 is S*N float2); convert in chunks
+    const long long chunk = 1LL << 22;
+    void *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)chunk * 16));
+    for (long long o = 0; o < n; o += chunk) {
+        const long long c = (n - o) < chunk ? (n - o) : chunk;
+        HIPCHK(hipMemcpyAsync(tmp, host + 2 * o, (size_t)c * 16, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL((k_from_f64<float>), dim3((unsigned)((c + 255) / 256)), dim3(256), 0, h->stream,
+                           (const double2 *)tmp, (float2 *)dst + o, c);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+    return PFDD_OK;
+}
+
+int download_field_f64(pfdd_gl *h, const void *src, double *host)
+{
+    const long long n = (long long)h->S * h->N;
+    if (h->prec == PFDD_FP64) {
+        HIPCHK(hipMemcpyAsync(host, src, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PFDD_OK;
+    }
+    const long long chunk = 1LL << 22;
+    void *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)chunk * 16));
+    for (long long o = 0; o < n; o += chunk) {
+        const long long c = (n - o) < chunk ? (n - o) : chunk;
+        hipLaunchKernelGGL((k_to_f64<float>), dim3((unsigned)((c + 255) / 256)), dim3(256), 0, h->stream,
+                           (const float2 *)src + o, (double2 *)tmp, c);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host + 2 * o, tmp, (size_t)c * 16, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+    return PFDD_OK;
+}
+
+} // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int pfdd_gl_version(void) { return (0 << 16) | 1; }
+
+const char *pfdd_gl_last_error(void) { return g_err.c_str(); }
+
+int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
+{
+    g_err.clear();
+    if (!d || !out) return fail(PFDD_ERR_ARG, "null descriptor or output");
+    *out = nullptr;
+    if (!size_supported(d->nx) || !size_supported(d->ny) || !size_supported(d->nz))
+        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each dimension must be one of 8,16,32,48,64,96,128,256,512",
+                    d->nx, d->ny, d->nz);
+    if (!systems_supported(d->ns))
+        return fail(PFDD_ERR_UNSUPPORTED, "slip_systems=%d: supported counts are 1,2,3,4,6,12", d->ns);
+    if (d->precision != PFDD_FP64 && d->precision != PFDD_FP32)
+        return fail(PFDD_ERR_ARG, "precision must be PFDD_FP64 or PFDD_FP32");
+    const bool ok_mid = d->precision == PFDD_FP64 ? mid_supported<double>(d->nx, d->ns)
+                                                  : mid_supported<float>(d->nx, d->ns);
+    if (!ok_mid)
+        return fail(PFDD_ERR_UNSUPPORTED, "nx=%d with %d systems exceeds the x-pass LDS tile", d->nx, d->ns);
+    if (!d->xn || !d->xb) return fail(PFDD_ERR_ARG, "xn/xb are required");
+    if (!(d->C44 > 0) || !(d->dslip != 0)) return fail(PFDD_ERR_ARG, "C44 must be > 0 and dslip != 0");
+
+    pfdd_gl *h = new pfdd_gl();
+    h->nx = d->nx;
+    h->ny = d->ny;
+    h->nz = d->nz;
+    h->S = d->ns;
+    h->prec = d->precision;
+    h->N = (long long)d->nx * d->ny * d->nz;
+    h->esz = d->precision == PFDD_FP64 ? 16 : 8;
+    h->device = d->device;
+    for (int q = 0; q < 9; ++q) h->four[q] = d->four[q];
+    for (int q = 0; q < 3 * d->ns; ++q) {
+        h->xn[q] = d->xn[q];
+        h->xbv[q] = d->xb[q];
+    }
+    h->C44 = d->C44;
+    h->C12 = d->C12;
+    h->C11 = d->C11;
+    h->S12 = d->S12;
+    h->S44 = d->S44;
+    h->dslip = d->dslip;
+    // Voigt moduli as Bmatrix recomputes them (fft_fftw_slab.cpp:487-490)
+    h->mu = d->C44 - (2.0 * d->C44 + d->C12 - d->C11) / 5.0;
+    const double ll = d->C12 - (2.0 * d->C44 + d->C12 - d->C11) / 5.0;
+    const double young = h->mu * (3 * ll + 2 * h->mu) / (ll + h->mu);
+    h->xnu = young / 2.0 / h->mu - 1.0;
+    h->P.S = d->ns;
+    h->P.cddt = d->cddt;
+    h->P.invN = 1.0 / (double)h->N;
+
+    auto bail = [&](int rc) {
+        destroy_all(h);
+        return rc;
+    };
+    int rc = set_coef(h, d->core_model, d->ncoef, d->coef, d->amp != nullptr);
+    if (rc) return bail(rc);
+    hipError_t e = hipSetDevice(d->device);
+    if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipSetDevice(%d): %s", d->device, hipGetErrorString(e)));
+    if (d->stream) {
+        h->stream = (hipStream_t)d->stream;
+    } else {
+        e = hipStreamCreate(&h->stream);
+        if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+        h->own_stream = true;
+    }
+    const size_t fbytes = (size_t)h->S * h->N * h->esz;
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        hipError_t ee = hipMalloc(p, bytes);
+        if (ee != hipSuccess)
+            return fail(PFDD_ERR_HIP, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(ee));
+        return PFDD_OK;
+    };
+    if ((rc = alloc(&h->xi[0], fbytes)) || (rc = alloc(&h->xi[1], fbytes)) || (rc = alloc(&h->T1, fbytes)))
+        return bail(rc);
+    e = hipMemsetAsync(h->xi[0], 0, fbytes, h->stream);
+    if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
+    if ((rc = upload_twiddles(h))) return bail(rc);
+
+    // per-block partials
+    const long long n5 = (long long)h->S * h->nx * h->ny;
+    const int nl5 = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
+    h->nb5 = (int)((n5 + nl5 - 1) / nl5);
+    h->nb3 = h->prec == PFDD_FP64 ? mid_blocks<double>(h->nx, h->S, (long long)h->ny * h->nz)
+                                  : mid_blocks<float>(h->nx, h->S, (long long)h->ny * h->nz);
+    h->pstride = h->nb5 > h->nb3 ? h->nb5 : h->nb3;
+    if ((rc = alloc((void **)&h->partial, sizeof(double) * NSLOT * h->pstride))) return bail(rc);
+    if ((rc = alloc((void **)&h->res, sizeof(double) * NRES))) return bail(rc);
+    if ((rc = alloc((void **)&h->done, sizeof(int))) || (rc = alloc((void **)&h->steps, sizeof(int))))
+        return bail(rc);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+
+    // obstacle mask + quirk Q3 predecessor table
+    if (d->xo) {
+        const long long n = (long long)h->S * h->N;
+        std::vector<uint8_t> m(n);
+        std::vector<long long> pv(n);
+        long long last = -1;
+        bool any = false;
+        for (long long q = 0; q < n; ++q) {
+            m[q] = d->xo[q] != 0.0;
+            any |= m[q] != 0;
+            pv[q] = last;
+            if (!m[q]) last = q;
+        }
+        if (any) {
+            if ((rc = alloc((void **)&h->mask, n)) || (rc = alloc((void **)&h->prev, n * sizeof(long long))))
+                return bail(rc);
+            if (hipMemcpy(h->mask, m.data(), n, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(h->prev, pv.data(), n * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(PFDD_ERR_HIP, "mask upload failed"));
+        }
+    }
+    if (d->amp) {
+        const long long n = (long long)h->S * h->N;
+        if (h->prec == PFDD_FP64) {
+            if ((rc = alloc(&h->amp, n * sizeof(double)))) return bail(rc);
+            if (hipMemcpy(h->amp, d->amp, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(PFDD_ERR_HIP, "amp upload failed"));
+        } else {
+            std::vector<float> f(d->amp, d->amp + n);
+            if ((rc = alloc(&h->amp, n * sizeof(float)))) return bail(rc);
+            if (hipMemcpy(h->amp, f.data(), n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(PFDD_ERR_HIP, "amp upload failed"));
+        }
+    }
+    rc = h->prec == PFDD_FP64 ? build_B<double>(h) : build_B<float>(h);
+    if (rc) return bail(rc);
+    e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "setup: %s", hipGetErrorString(e)));
+    *out = h;
+    return PFDD_OK;
+}
+
+int pfdd_gl_destroy(pfdd_gl *h)
+{
+    if (!h) return PFDD_OK;
+    destroy_all(h);
+    return PFDD_OK;
+}
+
+int pfdd_gl_set_xi(pfdd_gl *h, const double *xi)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!xi) return fail(PFDD_ERR_ARG, "null xi");
+    h->stepped = false;
+    return upload_field_f64(h, xi, h->xi[h->cur]);
+}
+
+int pfdd_gl_get_xi(pfdd_gl *h, double *xi)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!xi) return fail(PFDD_ERR_ARG, "null xi");
+    return download_field_f64(h, h->xi[h->cur], xi);
+}
+
+int pfdd_gl_xi_device(pfdd_gl *h, void **dptr)
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!dptr) return fail(PFDD_ERR_ARG, "null output");
+    *dptr = h->xi[h->cur];
+    return PFDD_OK;
+}
+
+int pfdd_gl_set_xi_device(pfdd_gl *h, const void *src)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!src) return fail(PFDD_ERR_ARG, "null source");
+    HIPCHK(hipMemcpyAsync(h->xi[h->cur], src, (size_t)h->S * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->stepped = false;
+    return PFDD_OK;
+}
+
+int pfdd_gl_set_tau(pfdd_gl *h, const double *tau)
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!tau) return fail(PFDD_ERR_ARG, "null tau");
+    for (int a = 0; a < h->S; ++a) h->P.tau[a] = tau[a];
+    return PFDD_OK;
+}
+
+int pfdd_gl_set_core_coef(pfdd_gl *h, int ncoef, const double *coef)
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    return set_coef(h, PFDD_CORE_SERIES, ncoef, coef, h->amp != nullptr);
+}
+
+int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (nsteps < 0) return fail(PFDD_ERR_ARG, "nsteps < 0");
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    HIPCHK(hipGetLastError());
+    const int cur0 = h->cur;
+    for (int s = 0; s < nsteps; ++s) {
+        rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
+        if (rc) return rc;
+    }
+    if (!out && tol <= 0) return PFDD_OK;
+    double res[NRES];
+    int done = 0, steps = 0;
+    HIPCHK(hipMemcpyAsync(res, h->res, sizeof(res), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&steps, h->steps, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the field moved steps times from buffer cur0
+    h->cur = (steps % 2) ? 1 - cur0 : cur0;
+    if (done == 2)
+        return fail(PFDD_ERR_NONFINITE, "non-finite reduction after %d GL iterations (field diverged)", steps);
+    if (out) {
+        out->xinorm = res[0];
+        out->xiave = res[1];
+        out->e_elastic = res[2];
+        out->e_core = res[3];
+        out->e_stress = res[4];
+        out->e_total = res[5];
+        out->e_core_ref = res[6];
+        out->steps_done = steps;
+        out->converged = done == 1;
+    }
+    return PFDD_OK;
+}
+
+int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!sum_re) return fail(PFDD_ERR_ARG, "null output");
+    const int nb = 256;
+    double *part = nullptr;
+    HIPCHK(hipMalloc(&part, sizeof(double) * nb * h->S));
+    if (h->prec == PFDD_FP64)
+        hipLaunchKernelGGL((k_sys_sums<double>), dim3(nb, h->S), dim3(NTHR), 0, h->stream,
+                           (const double2 *)h->xi[h->cur], h->N, part);
+    else
+        hipLaunchKernelGGL((k_sys_sums<float>), dim3(nb, h->S), dim3(NTHR), 0, h->stream,
+                           (const float2 *)h->xi[h->cur], h->N, part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hp((size_t)nb * h->S);
+    HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb * h->S, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(part));
+    for (int a = 0; a < h->S; ++a) {
+        double s = 0.0, c = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            const double y = hp[(size_t)a * nb + b] - c;
+            const double t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+        sum_re[a] = s;
+    }
+    return PFDD_OK;
+}
+
+
+int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], double ave_sigma[9],
+                        double *data_eps, double *data_sigma)
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!sigma_rot || !ave_eps || !ave_sigma) return fail(PFDD_ERR_ARG, "null argument");
+    if (data_eps || data_sigma)
+        return fail(PFDD_ERR_UNSUPPORTED, "field output of the diagnostics is not available in this build");
+    std::vector<double> sums(h->S);
+    if ((rc = pfdd_gl_xi_sums(h, sums.data()))) return rc;
+    const double nsize = (double)h->N;
+    // stressfree_strain (fft_fftw_slab.cpp:3252-3326): ave_epsd, avepst, avepsts
+    double ave_epsd[9] = {0}, avepst[9] = {0}, avepsts[9] = {0};
+    for (int a = 0; a < h->S; ++a)
+        for (int q = 0; q < 9; ++q) ave_epsd[q] += h->eps[a * 9 + q] * sums[a];
+    for (int q = 0; q < 9; ++q) ave_epsd[q] /= nsize;
+    auto Sijkl = [&](int i, int j, int k, int l) {
+        return h->S44 / 4 * ((i == k) * (j == l) + (i == l) * (j == k)) + h->S12 * (i == j) * (k == l);
+    };
+    auto Cijkl = [&](int i, int j, int k, int l) {
+        return h->C44 * ((i == k) * (j == l) + (i == l) * (j == k)) + h->C12 * (i == j) * (k == l);
+    };
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) {
+                    avepst[3 * i + j] += Sijkl(i, j, k, l) * sigma_rot[3 * k + l] * h->mu;
+                    avepsts[3 * i + j] += Sijkl(i, j, k, l) * sigma_rot[3 * k + l] * h->mu;
+                }
+            avepst[3 * i + j] += ave_epsd[3 * i + j];
+        }
+    // strain() + total_average_strain(): the k-space strain field has zero mean
+    // (F(0) = 0), so the average total strain is the symmetric part of avepst.
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ave_eps[3 * i + j] = (avepst[3 * i + j] + avepst[3 * j + i]) / 2.0;
+    // stress() (:3474-3553): mean of C:(strain - eps_d - avepsts) + sigma_rot mu
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < 3; ++m)
+                for (int l = 0; l < 3; ++l)
+                    s += Cijkl(i, j, m, l) * (avepst[3 * m + l] - ave_epsd[3 * m + l] - avepsts[3 * m + l]);
+            ave_sigma[3 * i + j] = s + sigma_rot[3 * i + j] * h->mu;
+        }
+    return PFDD_OK;
+}
+
+int pfdd_gl_diag(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], double ave_sigma[9])
+{
+    return pfdd_gl_diag_fields(h, sigma_rot, ave_eps, ave_sigma, nullptr, nullptr);
+}
+
+int pfdd_gl_profile(pfdd_gl *h, int nsteps, double *ms, int *nkernels)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!ms || nsteps <= 0) return fail(PFDD_ERR_ARG, "need ms[7] and nsteps > 0");
+    hipEvent_t ev[7];
+    for (int q = 0; q < 7; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    for (int s = 0; s < nsteps; ++s) {
+        rc = h->prec == PFDD_FP64 ? step_once<double>(h, 0.0, ev) : step_once<float>(h, 0.0, ev);
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(ev[6]));
+        for (int q = 0; q < 6; ++q) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+            acc[q] += t;
+        }
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[6]));
+        acc[6] += t;
+    }
+    for (int q = 0; q < 7; ++q) {
+        ms[q] = acc[q] / nsteps;
+        hipEventDestroy(ev[q]);
+    }
+    if (nkernels) *nkernels = 6;
+    return PFDD_OK;
+}
+
+int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step)
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    const double f = (double)h->S * h->N * h->esz;
+    const double bsz = (double)h->S * (h->S + 1) / 2 * h->N * (h->prec == PFDD_FP64 ? 8 : 4);
+    const double k[5] = {2 * f, 2 * f, 2 * f + bsz, 2 * f, 3 * f};
+    double tot = 0;
+    for (int q = 0; q < 5; ++q) {
+        if (bytes_per_kernel) bytes_per_kernel[q] = k[q];
+        tot += k[q];
+    }
+    if (bytes_per_step) *bytes_per_step = tot;
+    return PFDD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,229 @@
+"""Python binding of the gfx950 GL solver C ABI (include/pfdd_gl.h) via ctypes.
+
+The solver itself is the HIP library ``lib/libpfdd_gl.so`` of this package;
+this module only marshals arguments.  There is no CPU fallback: if the
+library is missing, loading fails with an error that says how to build it.
+
+Fields use the reference layout (``fft->xi[0]``, lanl PFDD-2.x
+src/fft_fftw_slab.cpp:1760-1762): complex128 arrays of shape (S, nx, ny, nz).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfdd_gl.so")
+
+PFDD_FP64, PFDD_FP32 = 0, 1
+CORE_NONE, CORE_PERFECT, CORE_SINE, CORE_MPEA, CORE_SERIES = 0, 1, 3, 7, 8
+MAX_SYSTEMS = 12
+STATUS = {0: "PFDD_OK", 1: "PFDD_ERR_ARG", 2: "PFDD_ERR_UNSUPPORTED", 3: "PFDD_ERR_HIP",
+          4: "PFDD_ERR_NONFINITE", 5: "PFDD_ERR_STATE"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class PfddError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GLDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("ns", ctypes.c_int), ("precision", ctypes.c_int),
+        ("four", ctypes.c_double * 9),
+        ("C44", ctypes.c_double), ("C12", ctypes.c_double), ("C11", ctypes.c_double),
+        ("S12", ctypes.c_double), ("S44", ctypes.c_double),
+        ("dslip", ctypes.c_double),
+        ("xn", _dp), ("xb", _dp),
+        ("cddt", ctypes.c_double),
+        ("core_model", ctypes.c_int), ("ncoef", ctypes.c_int),
+        ("coef", _dp), ("amp", _dp), ("xo", _dp),
+        ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
+    ]
+
+
+class GLReduce(ctypes.Structure):
+    _fields_ = [
+        ("xinorm", ctypes.c_double), ("xiave", ctypes.c_double),
+        ("e_elastic", ctypes.c_double), ("e_core", ctypes.c_double),
+        ("e_stress", ctypes.c_double), ("e_total", ctypes.c_double),
+        ("e_core_ref", ctypes.c_double),
+        ("steps_done", ctypes.c_int), ("converged", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTS = [
+    "pfdd_gl_version", "pfdd_gl_last_error", "pfdd_gl_create", "pfdd_gl_destroy",
+    "pfdd_gl_set_xi", "pfdd_gl_get_xi", "pfdd_gl_xi_device", "pfdd_gl_set_xi_device",
+    "pfdd_gl_set_tau", "pfdd_gl_set_core_coef", "pfdd_gl_step", "pfdd_gl_xi_sums",
+    "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"PFDD GL HIP library not found at {path}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` from the repository root")
+    L = ctypes.CDLL(path)
+    vp, ip, c_int, c_double = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double
+    sig = {
+        "pfdd_gl_version": (c_int, []),
+        "pfdd_gl_last_error": (ctypes.c_char_p, []),
+        "pfdd_gl_create": (c_int, [ctypes.POINTER(GLDesc), ctypes.POINTER(vp)]),
+        "pfdd_gl_destroy": (c_int, [vp]),
+        "pfdd_gl_set_xi": (c_int, [vp, _dp]),
+        "pfdd_gl_get_xi": (c_int, [vp, _dp]),
+        "pfdd_gl_xi_device": (c_int, [vp, ctypes.POINTER(vp)]),
+        "pfdd_gl_set_xi_device": (c_int, [vp, vp]),
+        "pfdd_gl_set_tau": (c_int, [vp, _dp]),
+        "pfdd_gl_set_core_coef": (c_int, [vp, c_int, _dp]),
+        "pfdd_gl_step": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
+        "pfdd_gl_xi_sums": (c_int, [vp, _dp]),
+        "pfdd_gl_diag": (c_int, [vp, _dp, _dp, _dp]),
+        "pfdd_gl_diag_fields": (c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "pfdd_gl_profile": (c_int, [vp, c_int, _dp, ip]),
+        "pfdd_gl_traffic": (c_int, [vp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise PfddError(rc, load_library().pfdd_gl_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class GLSolver:
+    """One grid of S order parameters on one GPU (drop-in for the FFTW_Slab GL path)."""
+
+    def __init__(self, shape, xn, xb, four, C44, C12, C11, dslip, cddt, *, S12=0.0, S44=0.0,
+                 core_model=CORE_NONE, coef=None, ncoef=0, amp=None, xo=None,
+                 precision=PFDD_FP64, device=0):
+        L = load_library()
+        self.shape = tuple(int(v) for v in shape)
+        self._xn, self._xb = _f64(xn).reshape(-1, 3), _f64(xb).reshape(-1, 3)
+        self.S = self._xn.shape[0]
+        self.precision = precision
+        self._coef = _f64(coef).reshape(-1) if coef is not None else None
+        self._amp = _f64(amp).reshape(-1) if amp is not None else None
+        self._xo = _f64(xo).reshape(-1) if xo is not None else None
+        d = GLDesc()
+        d.nx, d.ny, d.nz = self.shape
+        d.ns = self.S
+        d.precision = precision
+        for q, v in enumerate(_f64(four).reshape(-1)):
+            d.four[q] = v
+        d.C44, d.C12, d.C11, d.S12, d.S44, d.dslip = C44, C12, C11, S12, S44, dslip
+        d.xn, d.xb = _ptr(self._xn), _ptr(self._xb)
+        d.cddt = cddt
+        d.core_model, d.ncoef = core_model, ncoef
+        d.coef, d.amp, d.xo = _ptr(self._coef), _ptr(self._amp), _ptr(self._xo)
+        d.device = device
+        d.stream = None
+        h = ctypes.c_void_p()
+        _check(L.pfdd_gl_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._lib = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pfdd_gl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ field I/O
+    def set_xi(self, xi):
+        xi = np.ascontiguousarray(xi, dtype=np.complex128)
+        assert xi.shape == (self.S,) + self.shape, xi.shape
+        _check(self._lib.pfdd_gl_set_xi(self._h, xi.ctypes.data_as(_dp)))
+
+    def get_xi(self):
+        out = np.empty((self.S,) + self.shape, dtype=np.complex128)
+        _check(self._lib.pfdd_gl_get_xi(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def xi_device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        _check(self._lib.pfdd_gl_xi_device(self._h, ctypes.byref(p)))
+        return p.value
+
+    def set_xi_device(self, ptr: int):
+        _check(self._lib.pfdd_gl_set_xi_device(self._h, ctypes.c_void_p(ptr)))
+
+    def set_tau(self, tau):
+        t = _f64(tau)
+        assert t.size == self.S
+        _check(self._lib.pfdd_gl_set_tau(self._h, _ptr(t)))
+
+    def set_core_coef(self, coef, ncoef):
+        self._coef = _f64(coef).reshape(-1)
+        _check(self._lib.pfdd_gl_set_core_coef(self._h, int(ncoef), _ptr(self._coef)))
+
+    # ------------------------------------------------------------- stepping
+    def step(self, nsteps=1, tol=0.0, sync=True):
+        r = GLReduce()
+        _check(self._lib.pfdd_gl_step(self._h, int(nsteps), float(tol),
+                                      ctypes.byref(r) if sync else None))
+        return r.as_dict() if sync else None
+
+    def xi_sums(self):
+        out = np.zeros(self.S)
+        _check(self._lib.pfdd_gl_xi_sums(self._h, _ptr(out)))
+        return out
+
+    def diag(self, sigma_rot):
+        ae, asg = np.zeros(9), np.zeros(9)
+        _check(self._lib.pfdd_gl_diag(self._h, _ptr(_f64(sigma_rot).reshape(-1)), _ptr(ae), _ptr(asg)))
+        return ae.reshape(3, 3), asg.reshape(3, 3)
+
+    def profile(self, nsteps=10):
+        ms = np.zeros(7)
+        nk = ctypes.c_int()
+        _check(self._lib.pfdd_gl_profile(self._h, int(nsteps), _ptr(ms), ctypes.byref(nk)))
+        names = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update", "reduce", "step"]
+        return dict(zip(names, ms.tolist()))
+
+    def traffic(self):
+        k = np.zeros(5)
+        tot = ctypes.c_double()
+        _check(self._lib.pfdd_gl_traffic(self._h, _ptr(k), ctypes.byref(tot)))
+        return k, tot.value

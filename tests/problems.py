@@ -1,0 +1,180 @@
+"""Seeded GL problems shared by the oracle tests and the GPU parity tests.
+
+Setup quantities are derived with the oracle's restatement of the reference
+(material.cpp:156-167, fft.cpp:101-169, app.cpp:383-431, app_fcc.cpp:288-313,
+fft_fftw_slab.cpp:242-274,466-638,737-769) so that the oracle and the GPU
+solver receive identical inputs.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "phase-field-dislocation-dynamics-pfdd-2.x_amd", "python"))
+
+import oracle  # noqa: E402
+
+S3 = math.sqrt(3.0)
+
+# AppFCC::set_slip, app_fcc.cpp:216-286 (before the rotation block)
+FCC_XN = np.array([[1, 1, 1]] * 3 + [[-1, 1, 1]] * 3 + [[1, -1, 1]] * 3 + [[1, 1, -1]] * 3,
+                  dtype=float) / S3
+FCC_XB = np.array([
+    [-0.5, 0.5, 0.0], [0.0, -0.5, 0.5], [0.5, 0.0, -0.5],
+    [-0.5, -0.5, 0.0], [0.5, 0.0, 0.5], [0.0, -0.5, 0.5],
+    [-0.5, -0.5, 0.0], [0.5, 0.0, -0.5], [0.0, -0.5, -0.5],
+    [-0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, -0.5, -0.5]])
+
+
+def basal3():
+    """Three systems on one plane n = z with unit b at 0, 120, 240 degrees."""
+    xn = np.array([[0.0, 0.0, 1.0]] * 3)
+    xb = np.array([[math.cos(t), math.sin(t), 0.0] for t in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)])
+    return xn, xb
+
+
+# Cu-like isotropic material of the benchmark configurations
+MU, NU = 48e9, 0.34
+YOUNG = 2 * MU * (1 + NU)
+A_LAT = 0.3615e-9
+B_MAG = A_LAT / math.sqrt(2.0)
+ISF, USF = 41e-3, 158e-3
+DSLIP = 0.8165  # material.cpp:166
+
+
+def material():
+    mat = oracle.material(MU, YOUNG)
+    mat["dslip"] = DSLIP
+    mat["b"] = B_MAG
+    mod = oracle.voigt(mat["C44"], mat["C12"], mat["C11"])
+    mat["mu_s"] = mod["mu"]  # material->mu after Bmatrix
+    mat["xnu"] = mod["xnu"]
+    return mat
+
+
+def core_coef_sine(mat):
+    """core_energy_sine normalisation, fft_fftw_slab.cpp:1396-1406 (Cn uses the
+    already normalised isf, as the reference does)."""
+    mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
+    isf = ISF / mdb
+    cn = (USF - (isf / 2.0)) / mdb
+    return np.array([isf, cn])
+
+
+def geometry(prim_vectors=((1, 0, 0), (0, 1, 0), (0, 0, 1)), scale=(1.0, 1.0, 1.0)):
+    p, ip, nv = oracle.prim(*prim_vectors)
+    four, nf, vol = oracle.set_scale(p, *scale)
+    return {"prim": p, "invprim": ip, "norvol": nv, "four": four, "normfour": nf}
+
+
+def tau_of(geo, sigma, xn, xb, dslip=DSLIP):
+    srot = oracle.rotate_stress(geo["prim"], geo["invprim"], geo["norvol"], sigma)
+    return oracle.resolss(srot, xn, xb, dslip), srot
+
+
+def shear_sigma(value_mu, plane="xz"):
+    s = np.zeros((3, 3))
+    i, j = {"xz": (0, 2), "xy": (0, 1), "yz": (1, 2)}[plane]
+    s[i, j] = s[j, i] = value_mu
+    return s
+
+
+def loop_field(shape, S, loops, noise, seed=576):
+    """Initial xi: Re xi_a = 1 inside the given discs, plus RANMAR noise
+    2*noise*(u - 1/2) on every system and cell (order a, i, j, k)."""
+    nx, ny, nz = shape
+    xi = np.zeros((S,) + tuple(shape), dtype=np.complex128)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    for (a, plane_axis, plane_idx, c0, c1, r) in loops:
+        if plane_axis == 2:
+            inside = (k == plane_idx) & ((i - c0) ** 2 + (j - c1) ** 2 <= r * r)
+        elif plane_axis == 0:
+            inside = (i == plane_idx) & ((j - c0) ** 2 + (k - c1) ** 2 <= r * r)
+        else:
+            inside = (j == plane_idx) & ((i - c0) ** 2 + (k - c1) ** 2 <= r * r)
+        xi[a][inside] = 1.0
+    if noise > 0:
+        rng = oracle.Ranmar(seed)
+        u = rng.fill(xi.size).reshape(xi.shape)
+        xi = xi + 2.0 * noise * (u - 0.5)
+    return xi
+
+
+def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigma_mu=0.00625,
+              core="sine", cddt=0.01, prim_vectors=((1, 0, 0), (0, 1, 0), (0, 0, 1)), xo=None,
+              seed=576, ncoef=4):
+    mat = material()
+    geo = geometry(prim_vectors)
+    if systems == "basal3":
+        xn0, xb0 = basal3()
+    elif systems == "fcc12":
+        xn0, xb0 = FCC_XN.copy(), FCC_XB.copy()
+    elif systems == "one":
+        xn0, xb0 = np.array([[0.0, 0.0, 1.0]]), np.array([[1.0, 0.0, 0.0]])
+    else:
+        raise ValueError(systems)
+    xn, xb = oracle.rotate_slip(xn0, xb0, geo["four"], geo["normfour"])
+    S = xn.shape[0]
+    nx, ny, nz = shape
+    if loops is None:
+        loops = [(0, 2, nz // 2, nx / 2, ny / 2, max(2, nx // 5))]
+    xi0 = loop_field(shape, S, loops, noise, seed)
+    sigma = shear_sigma(sigma_mu)
+    tau, srot = tau_of(geo, sigma, xn, xb)
+    if core == "sine":
+        flag, par, nm = 3, core_coef_sine(mat), 0
+        gpu_core = dict(core_model=3, coef=par, ncoef=0)
+    elif core == "perfect":
+        par = core_coef_sine(mat)[:1]
+        flag, nm = 1, 0
+        gpu_core = dict(core_model=1, coef=par, ncoef=0)
+    elif core == "series":
+        rng = np.random.default_rng(seed)
+        base = core_coef_sine(mat)
+        par = np.zeros((S, ncoef))
+        par[:, 0], par[:, 1] = base[0], base[1]
+        par *= 1.0 + 0.2 * (2 * rng.random((S, ncoef)) - 1)
+        if ncoef > 2:
+            par[:, 2:] = 0.1 * base[0] * (2 * rng.random((S, ncoef - 2)) - 1)
+        flag, nm = 8, ncoef
+        gpu_core = dict(core_model=8, coef=par.reshape(-1), ncoef=ncoef)
+    elif core == "none":
+        flag, par, nm = 0, np.zeros(1), 0
+        gpu_core = dict(core_model=0, coef=None, ncoef=0)
+    else:
+        raise ValueError(core)
+    return {
+        "shape": tuple(shape), "S": S, "mat": mat, "geo": geo, "xn": xn, "xb": xb, "xi0": xi0,
+        "tau": tau, "sigma": sigma, "sigma_rot": srot, "cddt": cddt, "core_flag": flag,
+        "core_par": np.atleast_1d(par).reshape(-1), "nm": nm, "gpu_core": gpu_core, "xo": xo,
+    }
+
+
+def oracle_B(case):
+    m = case["mat"]
+    BB, eps, mod = oracle.bmatrix(case["shape"], m["C44"], m["C12"], m["C11"], m["dslip"],
+                                  case["xn"], case["xb"], case["geo"]["four"])
+    return BB, eps, mod
+
+
+def oracle_state(case, BB=None):
+    if BB is None:
+        BB, _, _ = oracle_B(case)
+    return oracle.GLState(case["xi0"], BB, case["cddt"], case["core_flag"], case["core_par"],
+                          case["nm"], None, case["xo"])
+
+
+def gpu_solver(case, precision=0):
+    import pfdd_gl
+    m = case["mat"]
+    s = pfdd_gl.GLSolver(case["shape"], case["xn"], case["xb"], case["geo"]["four"], m["C44"],
+                         m["C12"], m["C11"], m["dslip"], case["cddt"], S12=m["S12"], S44=m["S44"],
+                         xo=case["xo"], precision=precision, **case["gpu_core"])
+    s.set_xi(case["xi0"])
+    s.set_tau(case["tau"])
+    return s

@@ -1,0 +1,191 @@
+"""GPU parity of the HIP GL step against the FP64 CPU oracle, through the C ABI.
+
+Tolerances (floating point, written per test):
+  FP64 plain path: xi, energies and xinorm within 1e-9 relative of the oracle
+                   after the stated number of iterations;
+  FP32 mode:       within 1e-5 relative.
+Relative error of a field = max|gpu - ref| / max|ref|.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def scal_rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def run_pair(case, nsteps, precision=0, tau_seq=None, check_every=None, tol_field=1e-9,
+             tol_scal=1e-9):
+    st = P.oracle_state(case)
+    g = P.gpu_solver(case, precision)
+    try:
+        for s in range(nsteps):
+            tau = case["tau"] if tau_seq is None else tau_seq[s]
+            if tau_seq is not None:
+                g.set_tau(tau)
+            r_ref = st.step(tau)
+            r_gpu = g.step(1)
+            if check_every and (s % check_every == 0 or s == nsteps - 1):
+                for key, gk in (("xinorm", "xinorm"), ("xiave", "xiave"), ("E_el", "e_elastic"),
+                                ("E_core", "e_core"), ("E_stress", "e_stress"),
+                                ("E_total", "e_total"), ("E_core_ref", "e_core_ref")):
+                    if abs(r_ref[key]) > 1e-12:
+                        assert scal_rel(r_gpu[gk], r_ref[key]) < tol_scal, (s, key, r_gpu[gk], r_ref[key])
+        xg = g.get_xi()
+        assert relerr(xg, st.xi) < tol_field, relerr(xg, st.xi)
+        return st, g, r_ref, r_gpu
+    finally:
+        g.close()
+
+
+def test_one_step_basal3_16():
+    _say("case: 16^3 x 3 basal, sine core, 1 step, FP64")
+    case = P.make_case(shape=(16, 16, 16), noise=1e-2)
+    run_pair(case, 1, check_every=1, tol_field=1e-12, tol_scal=1e-11)
+
+
+def test_loop_basal3_32_200steps():
+    _say("case: 32^3 x 3 basal loop r=6 + noise, 200 steps, FP64")
+    case = P.make_case(shape=(32, 32, 32), loops=[(0, 2, 16, 16, 16, 6)], noise=1e-3)
+    run_pair(case, 200, check_every=50)
+
+
+def test_fcc12_16_20steps():
+    _say("case: 16^3 x 12 fcc, 20 steps, FP64")
+    case = P.make_case(shape=(16, 16, 16), systems="fcc12", noise=1e-3, sigma_mu=0.008,
+                       loops=[(0, 2, 8, 8, 8, 4), (5, 0, 4, 8, 8, 3)])
+    run_pair(case, 20, check_every=5)
+
+
+@pytest.mark.parametrize("shape", [(48, 16, 32), (16, 96, 8), (8, 32, 48)])
+def test_mixed_radix_grids(shape):
+    _say(f"case: {shape} x 3 basal, 10 steps, FP64")
+    nx, ny, nz = shape
+    case = P.make_case(shape=shape, noise=1e-2, loops=[(1, 2, nz // 2, nx / 2, ny / 2, 3)])
+    run_pair(case, 10, check_every=3)
+
+
+def test_skewed_grid_masked_frank_read_like():
+    _say("case: 32x32x16 skewed (60 deg) grid, mask, 30 steps, FP64")
+    shape = (32, 32, 16)
+    S = 3
+    xo = np.zeros((S,) + shape)
+    xo[:, :, 1:16, :] = 1.0          # frozen band (quirk Q3 path)
+    xo[:, 4, 20, 8] = 1.0            # isolated pinning points
+    xo[:, 28, 20, 8] = 1.0
+    case = P.make_case(shape=shape, noise=1e-3, sigma_mu=0.01, xo=xo,
+                       prim_vectors=((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1)),
+                       loops=[(0, 2, 8, 16, 20, 6)])
+    st, g, r_ref, r_gpu = run_pair(case, 30, check_every=10)
+
+
+def test_series_core_and_ramp():
+    _say("case: 16x32x16 x 3, 4-term sine series, stress ramp, 40 steps, FP64")
+    case = P.make_case(shape=(16, 32, 16), noise=1e-2, core="series", ncoef=4)
+    taus = [case["tau"] * (s / 40.0) for s in range(40)]
+    run_pair(case, 40, tau_seq=taus, check_every=10)
+
+
+def test_fp32_mode_matches_fp64_oracle():
+    _say("case: 32^3 x 3 basal loop, 200 steps, FP32 mode vs FP64 oracle")
+    case = P.make_case(shape=(32, 32, 32), loops=[(0, 2, 16, 16, 16, 6)], noise=1e-3)
+    run_pair(case, 200, precision=1, check_every=50, tol_field=1e-5, tol_scal=1e-5)
+
+
+def test_device_convergence_predicate():
+    _say("case: 16^3 x 3, tol-driven early stop")
+    case = P.make_case(shape=(16, 16, 16), noise=1e-3, sigma_mu=0.0)
+    st = P.oracle_state(case)
+    norms = []
+    for _ in range(60):
+        norms.append(st.step(case["tau"])["xinorm"])
+    tol = norms[25] * 1.0000001  # first iteration below tol is #26 (index 25) if decreasing
+    first = next(i for i, v in enumerate(norms) if v < tol)
+    g = P.gpu_solver(case)
+    try:
+        r = g.step(60, tol=tol)
+        assert r["converged"] == 1 and r["steps_done"] == first + 1
+        st2 = P.oracle_state(case)
+        for _ in range(first + 1):
+            st2.step(case["tau"])
+        assert relerr(g.get_xi(), st2.xi) < 1e-9
+    finally:
+        g.close()
+
+
+def test_diag_averages_match_oracle():
+    _say("case: 16^3 x 3 diagnostics averages after 5 steps")
+    case = P.make_case(shape=(16, 16, 16), noise=1e-2, sigma_mu=0.004)
+    m = case["mat"]
+    BB, eps, mod = P.oracle_B(case)
+    st = P.oracle_state(case, BB)
+    g = P.gpu_solver(case)
+    try:
+        for _ in range(5):
+            st.step(case["tau"])
+            g.step(1)
+        FF = oracle.fmatrix(case["shape"], m["C44"], m["C12"], mod["mu"], mod["xnu"], eps,
+                            case["geo"]["four"])
+        d = oracle.diagnostics(st.data_fftw, st.xi, FF, eps, m, mod["mu"], case["sigma_rot"])
+        ae, asg = g.diag(case["sigma_rot"])
+        scale_e = np.max(np.abs(d["ave_eps"]))
+        scale_s = np.max(np.abs(d["ave_sigma"]))
+        # absolute floor: components that are round-off in the reference
+        assert np.max(np.abs(ae - d["ave_eps"])) < 1e-9 * scale_e
+        assert np.max(np.abs(asg - d["ave_sigma"])) < 1e-9 * scale_s
+        sums = g.xi_sums()
+        assert np.allclose(sums, st.xi.real.sum(axis=(1, 2, 3)), rtol=1e-11, atol=1e-9)
+    finally:
+        g.close()
+
+
+def test_nonfinite_is_reported():
+    _say("case: NaN in the initial field -> PFDD_ERR_NONFINITE")
+    import pfdd_gl
+    case = P.make_case(shape=(8, 8, 8), noise=0.0)
+    case["xi0"][0, 1, 2, 3] = np.nan
+    g = P.gpu_solver(case)
+    try:
+        with pytest.raises(pfdd_gl.PfddError) as ei:
+            g.step(3)
+        assert ei.value.code == 4
+    finally:
+        g.close()
+
+
+def test_full_size_c1_properties_fp64():
+    """128^3 x 12 (production size): size-independent properties of one GPU
+    run: translation covariance and monotone energy decrease at constant stress."""
+    _say("case: 128^3 x 12 fcc FP64, 3 steps, roll covariance + energy decrease")
+    shape = (128, 128, 128)
+    case = P.make_case(shape=shape, systems="fcc12", noise=1e-3, sigma_mu=400e6 / P.MU,
+                       loops=[(0, 2, 64, 64, 64, 12), (4, 0, 40, 60, 70, 9)])
+    g = P.gpu_solver(case)
+    try:
+        e = [g.step(1)["e_total"] for _ in range(3)]
+        x1 = g.get_xi()
+        assert e[0] > e[1] > e[2]
+        shift = (5, -7, 3)
+        g.set_xi(np.roll(case["xi0"], shift, axis=(1, 2, 3)))
+        for _ in range(3):
+            g.step(1)
+        x2 = g.get_xi()
+        assert relerr(x2, np.roll(x1, shift, axis=(1, 2, 3))) < 1e-12
+    finally:
+        g.close()

@@ -1,0 +1,293 @@
+"""Pins of the CPU oracle (oracle/pfdd_oracle.c) against what the mathematics
+fixes: the DFT definition (numpy, naive long-double DFT), the closed-form
+isotropic elastic kernel of a single slip system, finite differences of the
+energy, RANMAR's published test values, stress equilibrium and the analytic
+averages of the output step.  The reference ships no golden vectors for this
+path, so these are the checks that pin it (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import problems as P
+
+
+# ----------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(8, 16, 12), (6, 10, 9), (4, 48, 3), (5, 7, 8)])
+def test_fft3d_matches_numpy(shape):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f = oracle.fft3d(x, -1)
+    assert np.allclose(f, np.fft.fftn(x), rtol=0, atol=1e-12 * np.abs(x).sum())
+    b = oracle.fft3d(x, +1)
+    assert np.allclose(b, np.fft.ifftn(x) * x.size, rtol=0, atol=1e-12 * np.abs(x).sum())
+
+
+@pytest.mark.parametrize("n", [8, 12, 48, 96, 128, 7])
+def test_fft_line_matches_naive_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for sign in (-1, 1):
+        ref = oracle.dft1d_naive(x, sign)
+        got = oracle.fft3d(x.reshape(1, 1, n), sign).reshape(n)
+        assert np.max(np.abs(got - ref)) < 5e-15 * n * np.max(np.abs(x))
+        got = oracle.fft3d(x.reshape(n, 1, 1), sign).reshape(n)  # along the slow axis
+        assert np.max(np.abs(got - ref)) < 5e-15 * n * np.max(np.abs(x))
+
+
+def test_fft_impulse_and_plane_wave():
+    n = (8, 12, 6)
+    x = np.zeros(n, complex)
+    x[0, 0, 0] = 1.0
+    assert np.allclose(oracle.fft3d(x, -1), 1.0, atol=1e-15)
+    i, j, k = np.meshgrid(*[np.arange(v) for v in n], indexing="ij")
+    wave = np.exp(2j * np.pi * (3 * i / 8 + 5 * j / 12 + 1 * k / 6))
+    f = oracle.fft3d(wave, -1)
+    expect = np.zeros(n, complex)
+    expect[3, 5, 1] = np.prod(n)
+    assert np.allclose(f, expect, atol=1e-9)
+
+
+# ------------------------------------------------------------- setup pieces
+def test_reciprocal_matrix_and_frequencies():
+    geo = P.geometry(((1, 0, 0), (0.5, math.sqrt(3) / 2, 0), (0, 0, 1)))
+    sclprim = geo["prim"]  # scale 1
+    # reciprocal lattice: four[i] . a_j = delta_ij (a_j = columns of sclprim)
+    assert np.allclose(geo["four"] @ sclprim, np.eye(3), atol=1e-14)
+    fx, fy, fz = oracle.frec(4, 6, 8, np.eye(3))
+    assert fx[2, 0, 0] == -0.5 and fx[1, 0, 0] == 0.25 and fx[3, 0, 0] == -0.25
+    assert fy[0, 3, 0] == -0.5 and fz[0, 0, 5] == -3 / 8
+
+
+def test_material_constants():
+    m = oracle.material(P.MU, P.YOUNG)
+    assert m["C44"] == P.MU
+    assert abs(m["nu"] - P.NU) < 1e-12
+    # isotropy: C11 = C12 + 2 C44, Voigt mu equals C44 exactly
+    assert abs(m["C11"] - (m["C12"] + 2 * m["C44"])) <= 1e-6
+    v = oracle.voigt(m["C44"], m["C12"], m["C11"])
+    assert v["mu"] == m["C44"] and abs(v["xnu"] - P.NU) < 1e-12
+    # compliance is the inverse of the stiffness on symmetric tensors
+    C = oracle.stiffness(m["C44"], m["C12"]).reshape(9, 9)
+    S = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    S[i, j, k, l] = m["S44"] / 4 * ((i == k) * (j == l) + (i == l) * (j == k)) + \
+                        m["S12"] * (i == j) * (k == l)
+    eps = np.array([[1e-3, 2e-3, 0], [2e-3, -1e-3, 5e-4], [0, 5e-4, 3e-4]])
+    sig = (C @ eps.reshape(9)).reshape(3, 3)
+    back = (S.reshape(9, 9) @ sig.reshape(9)).reshape(3, 3)
+    assert np.allclose(back, eps, rtol=1e-10, atol=1e-15)
+
+
+# --------------------------------------------------------------------- B(k)
+def _single_system_closed_form(nvec, bvec, f, xnu, d):
+    """B(k) of one glide system (b perpendicular to n, |b| = |n| = 1), isotropic
+    elasticity: (1/d^2)[1 - (n.k)^2 - (b.k)^2 + 2 (b.k)^2 (n.k)^2 / (1 - nu)]."""
+    f2 = (f ** 2).sum(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kh = f / np.sqrt(f2)
+    nk = np.tensordot(nvec, kh, 1)
+    bk = np.tensordot(bvec, kh, 1)
+    B = (1 - nk ** 2 - bk ** 2 + 2 * bk ** 2 * nk ** 2 / (1 - xnu)) / d ** 2
+    B[f2 == 0] = 0.0
+    return B
+
+
+@pytest.mark.parametrize("nb", [((0, 0, 1), (1, 0, 0)), ((1, 1, 1), (-1, 1, 0))])
+def test_bmatrix_single_system_closed_form(nb):
+    n = np.array(nb[0], float); n /= np.linalg.norm(n)
+    b = np.array(nb[1], float); b /= np.linalg.norm(b)
+    m = P.material()
+    shape = (8, 12, 16)
+    BB, eps, mod = oracle.bmatrix(shape, m["C44"], m["C12"], m["C11"], P.DSLIP, n[None], b[None], np.eye(3))
+    f = np.stack(oracle.frec(*shape, np.eye(3)))
+    expect = _single_system_closed_form(n, b, f, mod["xnu"], P.DSLIP)
+    assert np.max(np.abs(BB[0, 0] - expect)) < 1e-12 * np.max(np.abs(expect))
+    assert BB[0, 0, 0, 0, 0] == 0.0
+
+
+def _contracted_B(xn, xb, f, m, xnu, d):
+    """mu B_ab = sig_a:eps_b - [2 u_a.u_b/f^2 - (u_a.f)(u_b.f)/((1-nu) f^4)]/(2 C44), u_a = sig_a f."""
+    S = xn.shape[0]
+    eps = np.einsum("ai,aj->aij", xb, xn) / d
+    tr = np.trace(eps, axis1=1, axis2=2)
+    sig = m["C44"] * (eps + eps.transpose(0, 2, 1)) + m["C12"] * tr[:, None, None] * np.eye(3)
+    u = np.einsum("aij,j...->ai...", sig, f)
+    uf = np.einsum("ai...,i...->a...", u, f)
+    f2 = (f ** 2).sum(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        B = np.einsum("aij,bij->ab", sig, eps)[:, :, None, None, None] - (
+            2 * np.einsum("ai...,bi...->ab...", u, u) / f2
+            - uf[:, None] * uf[None, :] / ((1 - xnu) * f2 ** 2)) / (2 * m["C44"])
+    B = B / m["C44"]
+    B[:, :, f2 == 0] = 0.0
+    return B
+
+
+def test_bmatrix_fcc_skewed_invariants():
+    m = P.material()
+    geo = P.geometry(((1, 0, 0), (0.5, math.sqrt(3) / 2, 0), (0, 0, 1)))
+    xn, xb = oracle.rotate_slip(P.FCC_XN, P.FCC_XB, geo["four"], geo["normfour"])
+    shape = (6, 8, 4)
+    BB, eps, mod = oracle.bmatrix(shape, m["C44"], m["C12"], m["C11"], P.DSLIP, xn, xb, geo["four"])
+    Bab = BB.transpose(1, 0, 2, 3, 4)  # [a][b] (BB is stored [b][a])
+    # symmetric in (a, b)
+    assert np.max(np.abs(Bab - Bab.transpose(1, 0, 2, 3, 4))) < 1e-13
+    # even in k: B(-k) = B(k) (off the Nyquist planes, where frec maps +-n/2 to -1/2)
+    neg = Bab[:, :, (-np.arange(6)) % 6][:, :, :, (-np.arange(8)) % 8][:, :, :, :, (-np.arange(4)) % 4]
+    keep = np.ones(shape, bool)
+    keep[3], keep[:, 4], keep[:, :, 2] = False, False, False
+    assert np.max(np.abs(Bab[:, :, keep] - neg[:, :, keep])) < 1e-13
+    # B(0) = 0 and positive semidefinite everywhere
+    assert np.all(Bab[:, :, 0, 0, 0] == 0)
+    w = np.linalg.eigvalsh(Bab.reshape(12, 12, -1).transpose(2, 0, 1))
+    assert w.min() > -1e-12
+    # independent contracted-vector form of the same kernel
+    f = np.stack(oracle.frec(*shape, geo["four"]))
+    ref = _contracted_B(xn, xb, f, m, mod["xnu"], P.DSLIP)
+    assert np.max(np.abs(Bab - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+# --------------------------------------------------------------- core energy
+@pytest.mark.parametrize("flag", [1, 3, 8])
+def test_core_energy_derivative_fd(flag):
+    rng = np.random.default_rng(3)
+    xi = (rng.random((2, 4, 4, 4)) * 2 - 0.5).astype(complex)
+    if flag == 8:
+        par, nm = rng.random(8) * 0.01, 4
+    else:
+        par, nm = np.array([0.004, 0.013]), 0
+    dE, _, _ = oracle.core_energy(xi, flag, par, nm)
+    h = 1e-6
+    _, _, ep = oracle.core_energy(xi + h, flag, par, nm)
+    _, _, em = oracle.core_energy(xi - h, flag, par, nm)
+    assert abs((ep - em) / (2 * h) - dE.sum()) < 1e-7 * np.abs(dE).sum()
+
+
+def test_core_energy_sine_closed_form():
+    isf, cn = 0.004, 0.013
+    xi = np.array([0.0, 0.25, 0.5, 0.125]).reshape(1, 1, 1, 4).astype(complex)
+    dE, eref, ev = oracle.core_energy(xi, 3, np.array([isf, cn]))
+    s = np.sin
+    expect = [0.0, isf * np.pi, 0.0, isf * np.pi * s(np.pi / 4) + cn * 2 * np.pi * s(np.pi / 2)]
+    assert np.allclose(dE.reshape(-1), expect, rtol=1e-14, atol=1e-15)
+    dens = isf * s(np.pi * xi.real) ** 2 + cn * s(2 * np.pi * xi.real) ** 2
+    assert abs(ev - dens.sum()) < 1e-15 and abs(eref - dens.sum() / 1) < 1e-15  # nx = 1
+
+
+# ------------------------------------------------------------ the full step
+def _energy(case, xi):
+    st = P.oracle_state(dict(case, xi0=xi), case["_BB"])
+    return st.step(case["tau"])["E_total"]
+
+
+def test_step_is_gradient_flow_of_total_energy():
+    case = P.make_case(shape=(8, 8, 8), noise=1e-2)
+    case["_BB"], _, _ = P.oracle_B(case)
+    st = P.oracle_state(case, case["_BB"])
+    x0 = st.xi.copy()
+    st.step(case["tau"])
+    force = (x0 - st.xi) / case["cddt"]  # = dE/dRe xi + i dE/dIm xi
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for _ in range(4):
+        a, i, j, k = (rng.integers(0, v) for v in (3, 8, 8, 8))
+        for comp in (1.0, 1j):
+            xp, xm = x0.copy(), x0.copy()
+            xp[a, i, j, k] += h * comp
+            xm[a, i, j, k] -= h * comp
+            fd = (_energy(case, xp) - _energy(case, xm)) / (2 * h)
+            got = force[a, i, j, k].real if comp == 1.0 else force[a, i, j, k].imag
+            assert abs(fd - got) < 2e-6 * max(1.0, abs(got)), (fd, got)
+
+
+def test_energy_decreases_at_constant_stress():
+    case = P.make_case(shape=(16, 16, 8), noise=1e-3, cddt=0.05)
+    st = P.oracle_state(case)
+    e = [st.step(case["tau"])["E_total"] for _ in range(30)]
+    assert np.all(np.diff(e) < 0)
+
+
+def test_step_invariant_under_unit_shift_and_translation():
+    case = P.make_case(shape=(8, 8, 8), noise=1e-2)
+    BB, _, _ = P.oracle_B(case)
+    a = P.oracle_state(case, BB)
+    b = P.oracle_state(dict(case, xi0=case["xi0"] + 1.0), BB)
+    c = P.oracle_state(dict(case, xi0=np.roll(case["xi0"], (2, 3, 1), axis=(1, 2, 3))), BB)
+    for _ in range(3):
+        a.step(case["tau"]), b.step(case["tau"]), c.step(case["tau"])
+    assert np.max(np.abs((b.xi - 1.0) - a.xi)) < 1e-12
+    assert np.max(np.abs(np.roll(a.xi, (2, 3, 1), axis=(1, 2, 3)) - c.xi)) < 1e-12
+
+
+def test_mask_quirk_q3_hand_example():
+    # one system, 1x1x4 grid, cell 2 frozen: xinorm picks up (xi_2 - old xi_1)^2
+    xi = np.array([0.1, 0.2, 0.7, 0.3], dtype=complex).reshape(1, 1, 1, 4)
+    xo = np.array([0.0, 0.0, 1.0, 0.0]).reshape(1, 1, 1, 4)
+    BB = np.zeros((1, 1, 1, 1, 4))
+    st = oracle.GLState(xi, BB, 0.1, 0, None, 0, None, xo)
+    tau = 0.5
+    out = st.step([tau])
+    new = xi.reshape(-1).real.copy()
+    for q in (0, 1, 3):
+        new[q] += 0.1 * tau
+    expect = (0.05 ** 2) * 3 + (0.7 - 0.2) ** 2
+    assert abs(out["xinorm"] - math.sqrt(expect)) < 1e-15
+    assert np.allclose(st.xi.reshape(-1).real, new, atol=1e-16)
+    assert abs(out["xiave"] - new.sum()) < 1e-15
+
+
+# --------------------------------------------------------------- diagnostics
+def test_diagnostics_averages_and_equilibrium():
+    case = P.make_case(shape=(8, 8, 8), noise=1e-2, sigma_mu=0.004)
+    m = case["mat"]
+    BB, eps, mod = P.oracle_B(case)
+    st = P.oracle_state(case, BB)
+    st.step(case["tau"])
+    FF = oracle.fmatrix(case["shape"], m["C44"], m["C12"], mod["mu"], mod["xnu"], eps, case["geo"]["four"])
+    d = oracle.diagnostics(st.data_fftw, st.xi, FF, eps, m, mod["mu"], case["sigma_rot"], fields=True)
+    # analytic averages: sym(S:sigma mu + sum_a eps_a <Re xi_a>) and sigma mu
+    Sc = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    Sc[i, j, k, l] = m["S44"] / 4 * ((i == k) * (j == l) + (i == l) * (j == k)) + \
+                        m["S12"] * (i == j) * (k == l)
+    mean_xi = st.xi.real.mean(axis=(1, 2, 3))
+    ep = np.einsum("ijkl,kl->ij", Sc, case["sigma_rot"] * mod["mu"]) + np.einsum("aij,a->ij", eps, mean_xi)
+    assert np.allclose(d["ave_eps"], (ep + ep.T) / 2, rtol=1e-12, atol=1e-18)
+    assert np.allclose(d["ave_sigma"], case["sigma_rot"] * mod["mu"], rtol=1e-10, atol=1e-6)
+    # With strain and eigenstrain taken from the same field (the initial output
+    # step, App::setup: data_fftw = FFT(xi)), the stress field is in
+    # equilibrium: sum_j f_j sigmahat_ij(k) = 0 off k = 0 and the Nyquist planes.
+    xh = np.stack([oracle.fft3d(x, -1) for x in st.xi])
+    d = oracle.diagnostics(xh, st.xi, FF, eps, m, mod["mu"], case["sigma_rot"], fields=True)
+    sig = d["data_sigma"].real
+    sh = np.stack([np.stack([np.fft.fftn(sig[j, i]) for j in range(3)]) for i in range(3)])  # [i][j]
+    f = np.stack(oracle.frec(*case["shape"], case["geo"]["four"]))
+    div = np.einsum("ij...,j...->i...", sh, f)
+    keep = np.ones(case["shape"], bool)
+    keep[0, 0, 0] = False
+    keep[4], keep[:, 4], keep[:, :, 4] = False, False, False
+    assert np.max(np.abs(div[:, keep])) < 1e-9 * np.max(np.abs(sh[:, :, keep]))
+    # and the quirk-Q2 output step (pre-update strain) is not: the check has teeth
+    d2 = oracle.diagnostics(st.data_fftw, st.xi, FF, eps, m, mod["mu"], case["sigma_rot"], fields=True)
+    sh2 = np.stack([np.stack([np.fft.fftn(d2["data_sigma"].real[j, i]) for j in range(3)]) for i in range(3)])
+    div2 = np.einsum("ij...,j...->i...", sh2, f)
+    assert np.max(np.abs(div2[:, keep])) > 1e-6 * np.max(np.abs(sh2[:, :, keep]))
+
+
+# -------------------------------------------------------------------- RANMAR
+def test_ranmar_published_test_values():
+    # Marsaglia & Zaman test: seeds ij=1802, kl=9373; numbers 20001..20006 * 4096^2
+    seed = 1802 * 30082 + 9373 + 1
+    r = oracle.Ranmar(seed)  # RanMars::init already drew number 1
+    for _ in range(19999):
+        r.uniform()
+    got = [r.uniform() * 4096.0 * 4096.0 for _ in range(6)]
+    assert got == [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0]
