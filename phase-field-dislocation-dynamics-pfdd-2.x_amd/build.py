@@ -32,8 +32,17 @@ def build_lib():
     return out
 
 
+def build_host():
+    out = os.path.join(HERE, "lib", "libpfdd_host.so")
+    srcs = [os.path.join(HERE, "host", f) for f in ("pfdd_host.cpp", "pfdd_host_capi.cpp")]
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", out + ".tmp"] + srcs)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def main():
     build_lib()
+    build_host()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,102 @@
+// pfdd_host.hpp -- host-side (C++) mirror of the PFDD-2.x setup that feeds the
+// GL solver: material constants, primitive/reciprocal lattice, slip tables of
+// the app styles, stress rotation and resolved shear, RANMAR and the initial
+// order-parameter fields.  Everything here runs once per run or once per
+// stress state; the per-step work is on the GPU (include/pfdd_gl.h).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace pfdd_host {
+
+// Material::Material + set_constants (material.cpp:32-167, presets :176-835).
+struct Material {
+    double a = 0, b = 0, mu = 0, young = 0;
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, a1 = 0, a3 = 0;
+    double isf = 0, usf = 0, An = 0, Cn = 0;
+    double dslip = 0.8165;
+    // derived (material.cpp:156-167)
+    double C44 = 0, C12 = 0, C11 = 0, nu = 0, ll = 0, S11 = 0, S12 = 0, S44 = 0;
+    // Voigt re-derivation stored by Bmatrix (fft_fftw_slab.cpp:487-496)
+    double mu_s = 0, ll_s = 0, young_s = 0, xnu_s = 0;
+    std::vector<double> A_mpea; // per-cell USFE of core flag 7 (J/m^2), [a][i][j][k]
+
+    bool preset(const std::string &name); // Cu, Ni, Pd
+    void derive();
+};
+
+// Primitive vectors (fft.cpp:96-169) and reciprocal matrix (app.cpp:383-431).
+struct Geometry {
+    double prim[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double invprim[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double norvol = 1.0;
+    double sclprim[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double four[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double normfour[3] = {1, 1, 1};
+    double vol = 1.0;
+    bool primitive = false;
+    void set_primitive(const double x[3], const double y[3], const double z[3]);
+    void set_scale(double sa, double sb, double sc);
+};
+
+// Slip systems of an app style, before rotation into the grid frame.
+struct SlipTable {
+    std::string app;
+    int slip_systems = 0;
+    int num_planes = 0;
+    double CD = 0.5;       // dislocation mobility (app ctor)
+    double timestep = 1.0; // app->timestep
+    int default_core = 1;
+    bool rotate = true;         // set_slip applies the four/normfour rotation
+    std::vector<double> xn, xb; // S*3 each
+};
+
+// Known app styles: fcc, 3d-hcp-basal, frank_read, 3SBCC-like "fcc111-3"
+// (three <110> on one {111} plane).  Returns false for unknown names.
+bool slip_table(const std::string &app, SlipTable &t, double oflag = 0.0);
+
+// AppFCC::set_slip rotation block (app_fcc.cpp:288-313).
+void rotate_slip(SlipTable &t, const Geometry &g);
+// FFTW_Slab::rotate_stress (fft_fftw_slab.cpp:242-274).
+void rotate_stress(const Geometry &g, const double sigma[3][3], double sigma_rot[3][3]);
+// FFTW_Slab::resolSS_Schmid (fft_fftw_slab.cpp:737-769).
+void resolss(const SlipTable &t, const double sigma_rot[3][3], double dslip, double *tau);
+
+// Core-energy coefficients normalised as core_energy_* do (fft_fftw_slab.cpp:
+// 1165-1178, 1396-1406): flag 1 -> {An'}, flag 3 -> {isf', Cn'} with Cn' =
+// (usf - isf'/2)/(mu dslip b), the reference's own expression.
+std::vector<double> core_coefficients(const Material &m, int flag);
+
+// RanMars (random_mars.cpp:41-101).
+class RanMars {
+  public:
+    explicit RanMars(int seed);
+    double uniform();
+
+  private:
+    double u[98];
+    double c, cd, cm;
+    int i97, j97;
+};
+
+// Initial fields, reference layout xi[2*(k + j*nz + i*ny*nz + a*N) + {re,im}].
+struct Grid {
+    int nx, ny, nz, S;
+    long long N() const { return (long long)nx * ny * nz; }
+};
+// Seeded loops: Re xi_a = 1 on the cells of a Miller-plane disc
+// {h i + k j + l k = m0, |r - c| <= R} plus noise 2*amp*(u - 1/2) on every
+// cell (RANMAR, order a, i, j, k), then the loop draws from the same stream.
+void init_random_loops(const Grid &g, const SlipTable &t, int nloops, double rmin, double rmax,
+                       double noise, int seed, double *xi, std::vector<double> *loops_out = nullptr);
+// initial_sxtal_loop (fft_fftw_slab.cpp:3146-3197).
+void init_sxtal_loop(const Grid &g, double *xi);
+// initial_sxtal_frank_read (fft_fftw_slab.cpp:3032-3093): xi and the mask xo.
+void init_sxtal_frank_read(const Grid &g, double *xi, double *xo);
+// initial_sxtal_edge (fft_fftw_slab.cpp:3201-3247).
+void init_sxtal_edge(const Grid &g, double *xi);
+// initial_sxtal_3SBCC (fft_fftw_slab.cpp:3095-3144).
+void init_sxtal_3sbcc(const Grid &g, double *xi);
+
+} // namespace pfdd_host

@@ -1,0 +1,147 @@
+"""Host-side setup of GL problems through the C++ host helpers
+(include/pfdd_host.h, lib/libpfdd_host.so): material constants, slip tables,
+resolved shear stress, core-energy coefficients and seeded initial fields.
+This is the Python face of the same host layer the deck driver uses."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfdd_host.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class Material(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("a", "b", "mu", "young", "isf", "usf", "An", "Cn", "dslip", "C44", "C12", "C11",
+                 "nu", "S11", "S12", "S44", "mu_s", "xnu_s")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"host helper library missing at {LIB_PATH}; run the package build")
+        L = ctypes.CDLL(LIB_PATH)
+        ci, cd, cll, cp = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_char_p
+        mp = ctypes.POINTER(Material)
+        ip = ctypes.POINTER(ctypes.c_int)
+        for name, args in {
+            "pfdd_host_material_preset": [cp, mp],
+            "pfdd_host_material_derive": [mp],
+            "pfdd_host_slip": [cp, _dp, _dp, ip, _dp, _dp, _dp, _dp, _dp],
+            "pfdd_host_resolss": [cp, _dp, _dp, _dp, cd, _dp, _dp],
+            "pfdd_host_core_coef": [mp, ci, _dp, ip],
+            "pfdd_host_ranmar_fill": [ci, cll, _dp],
+            "pfdd_host_init_random_loops": [ci, ci, ci, cp, ci, cd, cd, cd, ci, _dp, _dp],
+            "pfdd_host_init_sxtal": [cp, ci, ci, ci, ci, _dp, _dp],
+        }.items():
+            f = getattr(L, name)
+            f.restype = ci
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise ValueError(f"{what} failed (rc={rc})")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def material(preset=None, **over):
+    m = Material()
+    _ok(lib().pfdd_host_material_preset(preset.encode() if preset else None, ctypes.byref(m)), "preset")
+    for k, v in over.items():
+        setattr(m, k, float(v))
+    _ok(lib().pfdd_host_material_derive(ctypes.byref(m)), "material")
+    return m
+
+
+def cu_like():
+    """Benchmark material: mu 48 GPa, nu 0.34, a 0.3615 nm, b = a/sqrt2, isf/usf 41/158 mJ/m^2."""
+    a = 0.3615e-9
+    return material(None, mu=48e9, young=2 * 48e9 * 1.34, a=a, b=a / np.sqrt(2.0), isf=41e-3, usf=158e-3)
+
+
+def slip(app, prim=None, scale=None):
+    xn, xb, four = np.zeros(36), np.zeros(36), np.zeros(9)
+    ns, cd, ts = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    pr = None if prim is None else np.ascontiguousarray(prim, dtype=np.float64).reshape(9)
+    sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+    _ok(lib().pfdd_host_slip(app.encode(), _p(pr), _p(sc), ctypes.byref(ns), _p(xn), _p(xb), _p(four),
+                             ctypes.byref(cd), ctypes.byref(ts)), "slip table " + app)
+    S = ns.value
+    return {"xn": xn[:3 * S].reshape(S, 3), "xb": xb[:3 * S].reshape(S, 3), "four": four.reshape(3, 3),
+            "CD": cd.value, "timestep": ts.value, "S": S}
+
+
+def resolved_shear(app, sigma_mu, dslip, prim=None, scale=None):
+    sl = slip(app, prim, scale)
+    tau, srot = np.zeros(sl["S"]), np.zeros(9)
+    pr = None if prim is None else np.ascontiguousarray(prim, dtype=np.float64).reshape(9)
+    sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+    _ok(lib().pfdd_host_resolss(app.encode(), _p(pr), _p(sc),
+                                _p(np.ascontiguousarray(sigma_mu, dtype=np.float64).reshape(9)),
+                                dslip, _p(tau), _p(srot)), "resolss")
+    return tau, srot.reshape(3, 3)
+
+
+def core_coef(m, flag):
+    c = np.zeros(8)
+    n = ctypes.c_int()
+    _ok(lib().pfdd_host_core_coef(ctypes.byref(m), int(flag), _p(c), ctypes.byref(n)), "core coef")
+    return c[:n.value].copy()
+
+
+def ranmar(seed, n):
+    out = np.empty(int(n))
+    _ok(lib().pfdd_host_ranmar_fill(int(seed), int(n), _p(out)), "ranmar")
+    return out
+
+
+def random_loops(shape, app, nloops, rmin, rmax, noise, seed):
+    sl = slip(app)
+    nx, ny, nz = shape
+    xi = np.empty((sl["S"], nx, ny, nz), dtype=np.complex128)
+    loops = np.zeros(5 * max(nloops, 1))
+    _ok(lib().pfdd_host_init_random_loops(nx, ny, nz, app.encode(), nloops, rmin, rmax, noise, seed,
+                                          xi.ctypes.data_as(_dp), _p(loops)), "init loops")
+    return xi, loops[:5 * nloops].reshape(nloops, 5)
+
+
+def sxtal(style, shape, S, with_mask=False):
+    nx, ny, nz = shape
+    xi = np.empty((S, nx, ny, nz), dtype=np.complex128)
+    xo = np.zeros((S, nx, ny, nz)) if with_mask else None
+    _ok(lib().pfdd_host_init_sxtal(style.encode(), nx, ny, nz, S, xi.ctypes.data_as(_dp), _p(xo)),
+        "init " + style)
+    return (xi, xo) if with_mask else xi
+
+
+def fcc_loops_problem(shape, nloops=8, rmin=8, rmax=20, noise=1e-3, shear_pa=400e6, seed=576,
+                      cddt=0.01):
+    """The 128^3 x 12 production workload: fcc table, Cu-like material, seeded
+    loops + noise, constant shear resolved on system 0 (sigma = tau0 (b0 n0 + n0 b0)),
+    sine core (flag 3)."""
+    m = cu_like()
+    sl = slip("fcc")
+    b0 = sl["xb"][0] / np.linalg.norm(sl["xb"][0])
+    n0 = sl["xn"][0] / np.linalg.norm(sl["xn"][0])
+    sigma = (shear_pa / m.mu) * (np.outer(b0, n0) + np.outer(n0, b0))
+    tau, srot = resolved_shear("fcc", sigma, m.dslip)
+    xi, loops = random_loops(shape, "fcc", nloops, rmin, rmax, noise, seed)
+    return {"shape": tuple(shape), "xn": sl["xn"], "xb": sl["xb"], "four": sl["four"],
+            "C44": m.C44, "C12": m.C12, "C11": m.C11, "S12": m.S12, "S44": m.S44, "dslip": m.dslip,
+            "mu_s": m.mu_s, "cddt": cddt, "core_coef": core_coef(m, 3), "tau": tau, "sigma_rot": srot,
+            "xi0": xi, "loops": loops, "material": m.as_dict()}
