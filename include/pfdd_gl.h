@@ -26,6 +26,11 @@ extern "C" {
 #define PFDD_GL_MAX_SYSTEMS 12
 #define PFDD_GL_MAX_COEF 8
 
+/* pfdd_gl_desc.flags */
+#define PFDD_GL_FLAG_STORED_B 1 /* build packed B(k) once and stream it every step (the
+                                   reference's BB array) instead of evaluating the Green's
+                                   function on the fly inside the x pass */
+
 enum pfdd_gl_status {
     PFDD_OK = 0,
     PFDD_ERR_ARG = 1,         /* invalid argument */
@@ -72,6 +77,7 @@ typedef struct pfdd_gl_desc {
     const double *xo;    /* optional ns*N obstacle mask fft->xo (nonzero = frozen)      */
     int device;          /* HIP device ordinal                                          */
     void *stream;        /* hipStream_t to order all work on; NULL = handle's own       */
+    int flags;           /* PFDD_GL_FLAG_* (0: on-the-fly B(k), FP64 arithmetic)         */
 } pfdd_gl_desc;
 
 /* Scalars of the last executed GL iteration (update_order_parameter,

@@ -201,25 +201,56 @@ __device__ __forceinline__ long long line_base(const LineMap &m, long long L)
 
 // Global -> LDS tile.  base[l] < 0 marks an inactive line.  Position-fastest
 // when the line is contiguous in memory (SP == 1), line-fastest otherwise
-// (then consecutive lines are contiguous: SD == 1).
+// (then consecutive lines are contiguous: SD == 1).  8-byte elements: every load of the thread is issued
+// before its first LDS write.  16-byte elements: a plain strided loop, which
+// measured faster than any batched form on gfx950.
 template <int N, int NL, int NTHR, typename V>
 __device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
                                           const long long *base, long long SP, int tid)
 {
-    constexpr int LDP = NL + 1, TOT = N * NL;
-    if (SP == 1) {
+    constexpr int LDP = NL + 1, TOT = N * NL, PER = (TOT + NTHR - 1) / NTHR;
+    if constexpr (sizeof(V) >= 16) {
+        if (SP == 1) {
 #pragma unroll 4
-        for (int e = tid; e < TOT; e += NTHR) {
-            const int l = e / N, p = e % N;
-            const long long b = base[l];
-            if (b >= 0) lds[p * LDP + l] = src[b + p];
+            for (int e = tid; e < TOT; e += NTHR) {
+                const int l = e / N, p = e % N;
+                const long long b = base[l];
+                if (b >= 0) lds[p * LDP + l] = src[b + p];
+            }
+        } else {
+#pragma unroll 4
+            for (int e = tid; e < TOT; e += NTHR) {
+                const int l = e % NL, p = e / NL;
+                const long long b = base[l];
+                if (b >= 0) lds[p * LDP + l] = src[b + p * SP];
+            }
         }
     } else {
-#pragma unroll 4
-        for (int e = tid; e < TOT; e += NTHR) {
-            const int l = e % NL, p = e / NL;
-            const long long b = base[l];
-            if (b >= 0) lds[p * LDP + l] = src[b + p * SP];
+        V r[PER];
+        if (SP == 1) {
+#pragma unroll
+            for (int t = 0; t < PER; ++t) {
+                const int e = tid + t * NTHR;
+                const long long b = (TOT % NTHR == 0 || e < TOT) ? base[e / N] : -1;
+                if (b >= 0) r[t] = src[b + e % N];
+            }
+#pragma unroll
+            for (int t = 0; t < PER; ++t) {
+                const int e = tid + t * NTHR;
+                if (TOT % NTHR == 0 || e < TOT) lds[(e % N) * LDP + e / N] = r[t];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < PER; ++t) {
+                const int e = tid + t * NTHR;
+                const long long b = (TOT % NTHR == 0 || e < TOT) ? base[e % NL] : -1;
+                if (b >= 0) r[t] = src[b + (e / NL) * SP];
+            }
+#pragma unroll
+            for (int t = 0; t < PER; ++t) {
+                const int e = tid + t * NTHR;
+                if (TOT % NTHR == 0 || e < TOT) lds[(e / NL) * LDP + e % NL] = r[t];
+            }
         }
     }
 }

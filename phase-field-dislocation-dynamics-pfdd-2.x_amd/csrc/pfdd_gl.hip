@@ -86,6 +86,17 @@ struct BParams {
     int nx, ny, nz, S;
 };
 
+// Constants of the on-the-fly Green's-function contraction of the x pass:
+// per system the symmetric stress sigma_a = C:eps_a (xx,yy,zz,xy,xz,yz) and
+// eps_a folded for a symmetric contraction (xx,yy,zz, xy+yx, xz+zx, yz+zy).
+struct GParams {
+    double sig6[MAXS * 6];
+    double eps6[MAXS * 6];
+    double four[9];
+    double inv2C44, inv1mnu, invmu;
+    int nx, ny, nz;
+};
+
 __device__ __forceinline__ double frac_freq(int kk, int n)
 {
     // frec(), fft_fftw_slab.cpp:294-322
@@ -160,6 +171,37 @@ __global__ void __launch_bounds__(NTHR)
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
     tile_store<N, NL, NTHR>(out, lds, bout, mo.SP, tid);
+}
+
+// Load the S lines of NP consecutive columns (T2[a][c0+q][0..N)) into the
+// tile, lines ordered (a, q); column-contiguous runs of N elements.
+template <int N, int NL, int NP, int NTHR, typename V>
+__device__ __forceinline__ void tile_load_cols(V *lds, const V *__restrict__ T2, long long c0, long long ncol,
+                                               long long ntot, int tid)
+{
+    constexpr int LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+    if constexpr (sizeof(V) >= 16) {
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e / N, p = e % N;
+            const int a = l / NP, q = l % NP;
+            if (c0 + q < ncol) lds[p * LDP + l] = T2[(long long)a * ntot + (c0 + q) * N + p];
+        }
+    } else {
+        V r[PER];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int e = tid + t * NTHR;
+            const int l = e / N, p = e % N;
+            const int a = l / NP, q = l % NP;
+            if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = T2[(long long)a * ntot + (c0 + q) * N + p];
+        }
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int e = tid + t * NTHR;
+            if (TOT % NTHR == 0 || e < TOT) lds[(e % N) * LDP + e / N] = r[t];
+        }
+    }
 }
 
 // --------------------------------------------- pass 3: x FFT + contraction
@@ -240,6 +282,165 @@ __global__ void __launch_bounds__(NTHR)
     if (tid == 0) part_el[blockIdx.x] = v[0];
 }
 
+
+// x pass with B(k) evaluated on the fly in FP64 instead of streamed from HBM.
+// With the complex stress of the transformed eigenstrain Sig = sum_b xihat_b
+// sigma_b, w = Sig f and u_a = sigma_a f,
+//   ghat_a = (1/mu) [ Sig:eps_a - (2 (u_a.w)/f^2 - (u_a.f)(f.w)/((1-nu) f^4)) / (2 C44) ]
+// which is sum_b B_ab(k) xihat_b for the kernel of Bmatrix (fft_fftw_slab.cpp:
+// 577-634) applied as internal_energy does (:1670-1695); ghat = 0 at f = 0
+// (B(0) = 0).  TPP threads share one k-point (wave-uniform split of the
+// systems), each producing S/TPP of the outputs.
+template <typename R, int N, int S> constexpr int otf_tpp()
+{
+    constexpr int pts = mid_pairs<R, N, S>() * N;
+    return (2 * pts == NTHR && S % 2 == 0) ? 2 : 1;
+}
+
+// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im).
+template <int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
+__device__ __forceinline__ void otf_point(const V *lds, int p, int q, const double f[3], double f2,
+                                          const double *__restrict__ cst, const GParams &G, Sink sink)
+{
+    // compiler-only barriers keep the (LDS-broadcast) constants from being
+    // hoisted and held in registers across points and systems
+    __asm__ volatile("" ::: "memory");
+    double Sr[6] = {0, 0, 0, 0, 0, 0}, Si[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < S; ++b) {
+        const V v = lds[p * LDP + b * NP + q];
+        const double xr = (double)v.x, xi = (double)v.y;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            Sr[c] += cst[b * 12 + c] * xr;
+            Si[c] += cst[b * 12 + c] * xi;
+        }
+    }
+    // w = Sig f (symmetric Sig: xx,yy,zz,xy,xz,yz)
+    const double wr[3] = {Sr[0] * f[0] + Sr[3] * f[1] + Sr[4] * f[2], Sr[3] * f[0] + Sr[1] * f[1] + Sr[5] * f[2],
+                          Sr[4] * f[0] + Sr[5] * f[1] + Sr[2] * f[2]};
+    const double wi[3] = {Si[0] * f[0] + Si[3] * f[1] + Si[4] * f[2], Si[3] * f[0] + Si[1] * f[1] + Si[5] * f[2],
+                          Si[4] * f[0] + Si[5] * f[1] + Si[2] * f[2]};
+    const double Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
+    const double Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
+    const double if2 = f2 > 0 ? 1.0 / f2 : 0.0;
+    const double c2 = 2.0 * if2 * G.inv2C44;
+    const double c4 = G.inv1mnu * if2 * if2 * G.inv2C44;
+#pragma unroll
+    for (int aa = 0; aa < SA; ++aa) {
+        __asm__ volatile("" ::: "memory");
+        const int a = H * SA + aa;
+        const double *sg = cst + a * 12, *ep = cst + a * 12 + 6;
+        const double u[3] = {sg[0] * f[0] + sg[3] * f[1] + sg[4] * f[2], sg[3] * f[0] + sg[1] * f[1] + sg[5] * f[2],
+                             sg[4] * f[0] + sg[5] * f[1] + sg[2] * f[2]};
+        const double uf = u[0] * f[0] + u[1] * f[1] + u[2] * f[2];
+        const double uwr = u[0] * wr[0] + u[1] * wr[1] + u[2] * wr[2];
+        const double uwi = u[0] * wi[0] + u[1] * wi[1] + u[2] * wi[2];
+        double ser = 0.0, sei = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            ser += Sr[c] * ep[c];
+            sei += Si[c] * ep[c];
+        }
+        double gr = (ser - (c2 * uwr - c4 * uf * Vr)) * G.invmu;
+        double gi = (sei - (c2 * uwi - c4 * uf * Vi)) * G.invmu;
+        if (!(f2 > 0)) gr = gi = 0.0;
+        sink(aa, gr, gi);
+    }
+}
+
+template <int N, int NP, int S>
+__device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p, double f[3], double &f2)
+{
+    const int k = (int)(col / G.ny), j = (int)(col % G.ny);
+    const double kx = frac_freq(p, G.nx), ky = frac_freq(j, G.ny), kz = frac_freq(k, G.nz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = kx * G.four[c] + ky * G.four[3 + c] + kz * G.four[6 + c];
+    f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+}
+
+template <typename R, int N, int S>
+__global__ void __launch_bounds__(NTHR)
+    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot,
+              const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
+              const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NP = mid_pairs<R, N, S>();
+    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
+    constexpr int TPP = otf_tpp<R, N, S>();
+    constexpr int SA = S / TPP;
+    constexpr int NPT = NP * N; // k-points per block
+    __shared__ V lds[N * LDP];
+    __shared__ double cst[S * 12];
+    __shared__ double red[NTHR / 64];
+    const int tid = threadIdx.x;
+    const long long c0 = (long long)blockIdx.x * NP;
+    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
+                                                                    : G.eps6[(c / 12) * 6 + c % 12 - 6];
+    tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
+    // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
+    // replaces xihat_a in the tile
+    double el = 0.0;
+    if constexpr (TPP == 1) {
+        // one thread per k-point: it reads all xihat_b of the point, then
+        // overwrites them with ghat_a
+        for (int pt = tid; pt < NPT; pt += NTHR) {
+            const int q = pt / N, p = pt % N;
+            if (c0 + q >= ncol) continue;
+            double f[3], f2;
+            otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
+            otf_point<S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, G, [&](int a, double gr, double gi) {
+                const int idx = p * LDP + a * NP + q;
+                const V x = lds[idx];
+                el += (double)x.x * gr + (double)x.y * gi;
+                lds[idx] = mk<V>((R)gr, (R)gi);
+            });
+        }
+    } else {
+        // two waves per set of k-points, each producing half of the systems;
+        // all reads of the tile finish (barrier) before any thread writes
+        const int h = tid / NPT, pt = tid % NPT;
+        const int q = pt / N, p = pt % N;
+        const bool ok = c0 + q < ncol;
+        double f[3], f2;
+        otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
+        double g[SA][2];
+        auto keep = [&](int aa, double gr, double gi) {
+            g[aa][0] = gr;
+            g[aa][1] = gi;
+        };
+        if (ok) {
+            if (__builtin_amdgcn_readfirstlane(h) == 0) otf_point<S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, G, keep);
+            else otf_point<S, SA, 1, NP, LDP>(lds, p, q, f, f2, cst, G, keep);
+        }
+        __syncthreads();
+        if (ok) {
+#pragma unroll
+            for (int aa = 0; aa < SA; ++aa) {
+                const int idx = p * LDP + (h * SA + aa) * NP + q;
+                const V x = lds[idx];
+                el += (double)x.x * g[aa][0] + (double)x.y * g[aa][1];
+                lds[idx] = mk<V>((R)g[aa][0], (R)g[aa][1]);
+            }
+        }
+    }
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+#pragma unroll 4
+    for (int e = tid; e < TOT; e += NTHR) {
+        const int l = e / N, p = e % N;
+        const int a = l / NP, q = l % NP;
+        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds[p * LDP + l];
+    }
+    double v[1] = {el};
+    block_sum<1, NTHR>(v, red, tid);
+    if (tid == 0) part_el[blockIdx.x] = v[0];
+}
+
 // ------------------------------------- pass 5: inverse z FFT + fused update
 // update_order_parameter (fft_fftw_slab.cpp:1746-1790) with the core-energy
 // derivative of core_energy_* evaluated on the old field, as the reference
@@ -273,21 +474,38 @@ __global__ void __launch_bounds__(NTHR)
     }
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
+    // the old field of this thread's output cells, fetched while the FFT runs
+    // (8-byte elements only: batching 16-byte loads measured slower)
+    constexpr int PER = (TOT + NTHR - 1) / NTHR;
+    constexpr bool PREF = sizeof(V) < 16;
+    V xold[PREF ? PER : 1];
+    if constexpr (PREF) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int e = tid + t * NTHR;
+            const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
+            xold[t] = b >= 0 ? xo[b + e % N] : mk<V>((R)0, (R)0);
+        }
+    }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
 
     const R cddt = (R)P.cddt, invN = (R)P.invN;
     const int nm = P.ncoef;
     double s_norm = 0.0, s_ave = 0.0, s_core = 0.0, s_str = 0.0;
-#pragma unroll 2
-    for (int e = tid; e < TOT; e += NTHR) {
+#pragma unroll(PREF ? PER : 2)
+    for (int t = 0; t < PER; ++t) {
+        const int e = tid + t * NTHR;
+        if (TOT % NTHR != 0 && e >= TOT) continue;
         const int l = e / N, p = e % N; // mo.SP == 1: rows of xi are contiguous
         const long long b = bout[l];
         if (b < 0) continue;
         const long long idx = b + p;
         const int a = lsys[l];
         const V g = lds[p * LDP + l];
-        const V x = xo[idx];
+        V x;
+        if constexpr (PREF) x = xold[t];
+        else x = xo[idx];
         // core energy density and its derivative at Re xi (sine series)
         R sp, cp;
         if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
@@ -337,24 +555,30 @@ __global__ void __launch_bounds__(NTHR)
     }
 }
 
-// Deterministic final reduction of the block partials (Kahan within each
-// thread's strided sweep, then a fixed-shape tree).  Writes the iteration's
-// scalars, counts the iteration and raises `done` on convergence or on a
-// non-finite result.
+// Deterministic reduction of the block partials in two levels: every block
+// sums a fixed chunk (Kahan within a thread, fixed-shape tree across the
+// block) into stage2; the last block to finish (atomic ticket) sums stage2 in
+// fixed order, writes the iteration's scalars, counts the iteration and
+// raises `done` on convergence or a non-finite result.
+constexpr int NB2 = 32;
+
 __global__ void __launch_bounds__(NTHR)
-    k_finalize(const double *__restrict__ partial, long long pstride, int nb5, int nb3, double invN,
-               double inv_nx, double tol, double *__restrict__ res, int *__restrict__ done,
-               int *__restrict__ steps)
+    k_reduce(const double *__restrict__ partial, long long pstride, int nb5, int nb3, double invN,
+             double inv_nx, double tol, double *__restrict__ stage2, unsigned *__restrict__ ticket,
+             double *__restrict__ res, int *__restrict__ done, int *__restrict__ steps)
 {
     if (*done) return;
     __shared__ double red[NSLOT * (NTHR / 64)];
+    __shared__ int last;
     const int tid = threadIdx.x;
     double v[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         const int nb = s == 4 ? nb3 : nb5;
+        const int chunk = (nb + NB2 - 1) / NB2;
+        const int lo = blockIdx.x * chunk, hi = min(nb, lo + chunk);
         double sum = 0.0, c = 0.0;
-        for (int b = tid; b < nb; b += NTHR) {
+        for (int b = lo + tid; b < hi; b += NTHR) {
             const double y = partial[s * pstride + b] - c;
             const double t = sum + y;
             c = (t - sum) - y;
@@ -364,20 +588,39 @@ __global__ void __launch_bounds__(NTHR)
     }
     block_sum<NSLOT, NTHR>(v, red, tid);
     if (tid == 0) {
-        const double xinorm = sqrt(v[0]);
-        const double e_el = 0.5 * v[4] * invN;
-        res[0] = xinorm;
-        res[1] = v[1];
-        res[2] = e_el;
-        res[3] = v[2];
-        res[4] = v[3];
-        res[5] = e_el + v[2] - v[3];
-        res[6] = v[2] * inv_nx;
-        bool finite = isfinite(xinorm) && isfinite(v[1]) && isfinite(e_el) && isfinite(v[2]);
-        res[7] = finite ? 0.0 : 1.0;
-        *steps += 1;
-        if (!finite || (tol > 0 && xinorm < tol)) *done = finite ? 1 : 2;
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) stage2[s * NB2 + blockIdx.x] = v[s];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == NB2 - 1;
     }
+    __syncthreads();
+    if (!last || tid != 0) return;
+    __threadfence();
+    double w[NSLOT];
+    for (int s = 0; s < NSLOT; ++s) {
+        double sum = 0.0, c = 0.0;
+        for (int b = 0; b < NB2; ++b) {
+            const double y = __builtin_nontemporal_load(&stage2[s * NB2 + b]) - c;
+            const double t = sum + y;
+            c = (t - sum) - y;
+            sum = t;
+        }
+        w[s] = sum;
+    }
+    *ticket = 0u;
+    const double xinorm = sqrt(w[0]);
+    const double e_el = 0.5 * w[4] * invN;
+    res[0] = xinorm;
+    res[1] = w[1];
+    res[2] = e_el;
+    res[3] = w[2];
+    res[4] = w[3];
+    res[5] = e_el + w[2] - w[3];
+    res[6] = w[2] * inv_nx;
+    const bool finite = isfinite(xinorm) && isfinite(w[1]) && isfinite(e_el) && isfinite(w[2]);
+    res[7] = finite ? 0.0 : 1.0;
+    *steps += 1;
+    if (!finite || (tol > 0 && xinorm < tol)) *done = finite ? 1 : 2;
 }
 
 __global__ void k_reset(int *done, int *steps)
@@ -565,6 +808,43 @@ hipError_t launch_mid(int n, int S, void *T2, const void *B, long long ncol, lon
     return hipErrorInvalidValue;
 }
 
+template <typename R, int S>
+hipError_t launch_mid_otf_s(int n, void *T2, const GParams &G, long long ncol, long long ntot, const void *tw,
+                            double *part_el, const int *done, hipStream_t st)
+{
+    typedef typename cvec<R>::type V;
+#define PFDD_CASE(n_)                                                                             \
+    if (n == n_) {                                                                                \
+        if constexpr (mid_fits<R, n_, S>()) {                                                     \
+            constexpr int NP = mid_pairs<R, n_, S>();                                             \
+            const unsigned nb = (unsigned)((ncol + NP - 1) / NP);                                 \
+            hipLaunchKernelGGL((k_mid_otf<R, n_, S>), dim3(nb), dim3(NTHR), 0, st, (V *)T2, G,    \
+                               ncol, ntot, (const V *)tw, part_el, done);                         \
+            return hipGetLastError();                                                             \
+        } else {                                                                                  \
+            return hipErrorInvalidValue;                                                          \
+        }                                                                                         \
+    }
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_mid_otf(int n, int S, void *T2, const GParams &G, long long ncol, long long ntot,
+                          const void *tw, double *part_el, const int *done, hipStream_t st)
+{
+    switch (S) {
+    case 1: return launch_mid_otf_s<R, 1>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 2: return launch_mid_otf_s<R, 2>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 3: return launch_mid_otf_s<R, 3>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 4: return launch_mid_otf_s<R, 4>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 6: return launch_mid_otf_s<R, 6>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 12: return launch_mid_otf_s<R, 12>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 } // namespace
 
 // ================================================================== handle
@@ -586,6 +866,10 @@ struct pfdd_gl {
     double *partial = nullptr;
     long long pstride = 0;
     int nb3 = 0, nb5 = 0;
+    double *stage2 = nullptr; // NSLOT * NB2
+    unsigned *ticket = nullptr;
+    bool stored_B = false;    // stream packed B(k) from HBM instead of evaluating it on the fly
+    GParams G{};
     double *res = nullptr; // NRES doubles
     int *done = nullptr;
     int *steps = nullptr;
@@ -680,8 +964,12 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    HIPCHK((launch_mid<R>(h->nx, h->S, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
-                          h->partial + 4 * h->pstride, h->done, st)));
+    if (h->stored_B)
+        HIPCHK((launch_mid<R>(h->nx, h->S, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
+                              h->partial + 4 * h->pstride, h->done, st)));
+    else
+        HIPCHK((launch_mid_otf<R>(h->nx, h->S, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
+                                  h->partial + 4 * h->pstride, h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
@@ -696,8 +984,9 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
                                     h->done, st);
     HIPCHK(e);
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTHR), 0, st, h->partial, h->pstride, h->nb5, h->nb3,
-                       1.0 / (double)h->N, 1.0 / (double)h->nx, tol, h->res, h->done, h->steps);
+    hipLaunchKernelGGL(k_reduce, dim3(NB2), dim3(NTHR), 0, st, h->partial, h->pstride, h->nb5, h->nb3,
+                       1.0 / (double)h->N, 1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done,
+                       h->steps);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[6], st));
     h->cur = 1 - h->cur;
@@ -723,7 +1012,7 @@ void destroy_all(pfdd_gl *h)
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
-                    h->amp, h->partial, h->res, h->done, h->steps};
+                    h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
@@ -796,6 +1085,22 @@ int build_B(pfdd_gl *h)
     bp.ny = h->ny;
     bp.nz = h->nz;
     bp.S = S;
+    GParams &G = h->G;
+    static const int si[6] = {0, 4, 8, 1, 2, 5}; // xx yy zz xy xz yz in row-major 3x3
+    for (int a = 0; a < S; ++a)
+        for (int c = 0; c < 6; ++c) {
+            const int q = si[c], r = q / 3, cc = q % 3;
+            G.sig6[a * 6 + c] = bp.sig[a * 9 + q];
+            G.eps6[a * 6 + c] = c < 3 ? bp.eps[a * 9 + q] : bp.eps[a * 9 + 3 * r + cc] + bp.eps[a * 9 + 3 * cc + r];
+        }
+    for (int q = 0; q < 9; ++q) G.four[q] = h->four[q];
+    G.inv2C44 = bp.inv2C44;
+    G.inv1mnu = bp.inv1mnu;
+    G.invmu = bp.invmu;
+    G.nx = h->nx;
+    G.ny = h->ny;
+    G.nz = h->nz;
+    if (!h->stored_B) return PFDD_OK;
     const long long np = (long long)S * (S + 1) / 2;
     HIPCHK(hipMalloc(&h->B, (size_t)np * h->N * sizeof(R)));
     const unsigned nb = (unsigned)((h->N + NTHR - 1) / NTHR);
@@ -904,6 +1209,7 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
     const double ll = d->C12 - (2.0 * d->C44 + d->C12 - d->C11) / 5.0;
     const double young = h->mu * (3 * ll + 2 * h->mu) / (ll + h->mu);
     h->xnu = young / 2.0 / h->mu - 1.0;
+    h->stored_B = (d->flags & PFDD_GL_FLAG_STORED_B) != 0;
     h->P.S = d->ns;
     h->P.cddt = d->cddt;
     h->P.invN = 1.0 / (double)h->N;
@@ -945,6 +1251,11 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
     h->pstride = h->nb5 > h->nb3 ? h->nb5 : h->nb3;
     if ((rc = alloc((void **)&h->partial, sizeof(double) * NSLOT * h->pstride))) return bail(rc);
     if ((rc = alloc((void **)&h->res, sizeof(double) * NRES))) return bail(rc);
+    if ((rc = alloc((void **)&h->stage2, sizeof(double) * NSLOT * NB2)) ||
+        (rc = alloc((void **)&h->ticket, sizeof(unsigned))))
+        return bail(rc);
+    e = hipMemsetAsync(h->ticket, 0, sizeof(unsigned), h->stream);
+    if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
     if ((rc = alloc((void **)&h->done, sizeof(int))) || (rc = alloc((void **)&h->steps, sizeof(int))))
         return bail(rc);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
@@ -1208,7 +1519,8 @@ int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step
     int rc;
     if ((rc = check_handle(h))) return rc;
     const double f = (double)h->S * h->N * h->esz;
-    const double bsz = (double)h->S * (h->S + 1) / 2 * h->N * (h->prec == PFDD_FP64 ? 8 : 4);
+    const double bsz =
+        h->stored_B ? (double)h->S * (h->S + 1) / 2 * h->N * (h->prec == PFDD_FP64 ? 8 : 4) : 0.0;
     const double k[5] = {2 * f, 2 * f, 2 * f + bsz, 2 * f, 3 * f};
     double tot = 0;
     for (int q = 0; q < 5; ++q) {

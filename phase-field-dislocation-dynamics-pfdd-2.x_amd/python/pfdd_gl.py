@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfdd_gl.so")
 PFDD_FP64, PFDD_FP32 = 0, 1
 CORE_NONE, CORE_PERFECT, CORE_SINE, CORE_MPEA, CORE_SERIES = 0, 1, 3, 7, 8
 MAX_SYSTEMS = 12
+FLAG_STORED_B = 1
 STATUS = {0: "PFDD_OK", 1: "PFDD_ERR_ARG", 2: "PFDD_ERR_UNSUPPORTED", 3: "PFDD_ERR_HIP",
           4: "PFDD_ERR_NONFINITE", 5: "PFDD_ERR_STATE"}
 
@@ -45,6 +46,7 @@ class GLDesc(ctypes.Structure):
         ("core_model", ctypes.c_int), ("ncoef", ctypes.c_int),
         ("coef", _dp), ("amp", _dp), ("xo", _dp),
         ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
+        ("flags", ctypes.c_int),
     ]
 
 
@@ -126,7 +128,7 @@ class GLSolver:
 
     def __init__(self, shape, xn, xb, four, C44, C12, C11, dslip, cddt, *, S12=0.0, S44=0.0,
                  core_model=CORE_NONE, coef=None, ncoef=0, amp=None, xo=None,
-                 precision=PFDD_FP64, device=0):
+                 precision=PFDD_FP64, device=0, stored_B=False):
         L = load_library()
         self.shape = tuple(int(v) for v in shape)
         self._xn, self._xb = _f64(xn).reshape(-1, 3), _f64(xb).reshape(-1, 3)
@@ -148,6 +150,7 @@ class GLSolver:
         d.coef, d.amp, d.xo = _ptr(self._coef), _ptr(self._amp), _ptr(self._xo)
         d.device = device
         d.stream = None
+        d.flags = FLAG_STORED_B if stored_B else 0
         h = ctypes.c_void_p()
         _check(L.pfdd_gl_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

@@ -169,12 +169,12 @@ def oracle_state(case, BB=None):
                           case["nm"], None, case["xo"])
 
 
-def gpu_solver(case, precision=0):
+def gpu_solver(case, precision=0, stored_B=False):
     import pfdd_gl
     m = case["mat"]
     s = pfdd_gl.GLSolver(case["shape"], case["xn"], case["xb"], case["geo"]["four"], m["C44"],
                          m["C12"], m["C11"], m["dslip"], case["cddt"], S12=m["S12"], S44=m["S44"],
-                         xo=case["xo"], precision=precision, **case["gpu_core"])
+                         xo=case["xo"], precision=precision, stored_B=stored_B, **case["gpu_core"])
     s.set_xi(case["xi0"])
     s.set_tau(case["tau"])
     return s

@@ -31,9 +31,9 @@ def scal_rel(a, b):
 
 
 def run_pair(case, nsteps, precision=0, tau_seq=None, check_every=None, tol_field=1e-9,
-             tol_scal=1e-9):
+             tol_scal=1e-9, stored_B=False):
     st = P.oracle_state(case)
-    g = P.gpu_solver(case, precision)
+    g = P.gpu_solver(case, precision, stored_B)
     try:
         for s in range(nsteps):
             tau = case["tau"] if tau_seq is None else tau_seq[s]
@@ -66,11 +66,19 @@ def test_loop_basal3_32_200steps():
     run_pair(case, 200, check_every=50)
 
 
-def test_fcc12_16_20steps():
-    _say("case: 16^3 x 12 fcc, 20 steps, FP64")
+@pytest.mark.parametrize("stored_B", [False, True])
+def test_fcc12_16_20steps(stored_B):
+    _say(f"case: 16^3 x 12 fcc, 20 steps, FP64, stored_B={stored_B}")
     case = P.make_case(shape=(16, 16, 16), systems="fcc12", noise=1e-3, sigma_mu=0.008,
                        loops=[(0, 2, 8, 8, 8, 4), (5, 0, 4, 8, 8, 3)])
-    run_pair(case, 20, check_every=5)
+    run_pair(case, 20, check_every=5, stored_B=stored_B)
+
+
+def test_stored_B_fp32_and_skewed():
+    _say("case: 16x16x32 x 3 skewed grid, stored packed B, FP32, 30 steps")
+    case = P.make_case(shape=(16, 16, 32), noise=1e-2,
+                       prim_vectors=((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1)))
+    run_pair(case, 30, precision=1, check_every=10, tol_field=1e-5, tol_scal=1e-5, stored_B=True)
 
 
 @pytest.mark.parametrize("shape", [(48, 16, 32), (16, 96, 8), (8, 32, 48)])
