@@ -85,17 +85,103 @@ def roofline(s, prof_steps=20):
     }
 
 
-def load_pmc_traffic(tag):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel)."""
+def load_pmc_traffic(tag, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), or None."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{tag}.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f)
+            k = json.load(f)["kernels"].get(kernel)
+        return k["hbm_bytes_per_launch"] if k else None
     except Exception:
         return None
+
+
+KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update": "inv_z_update"}
+
+
+def pmc_probe(size):
+    """Workload for the rocprofv3 --pmc runs: a plain element-wise copy of known
+    size (torch's element-wise multiply by 1: ordinary loads and stores, the
+    calibration kernel), then 3 GL iterations in each precision."""
+    import torch
+    n = 1 << 28  # float32 elements: 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(f"pmc-probe: copied {4 * n} bytes x3", flush=True)
+    del x, y
+    torch.cuda.empty_cache()
+    prob = make_workload(size)
+    for prec in (pfdd_gl.PFDD_FP64, pfdd_gl.PFDD_FP32):
+        s = solver_for(prob, prec)
+        s.step(3)
+        s.close()
+        print(f"pmc-probe: precision {prec} done", flush=True)
+    return 0
+
+
+def pmc_summarize(fetch_dir, write_dir, size, out_dir):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) from two rocprofv3 --pmc runs
+    of --pmc-probe, calibrated by the copy kernel, for the unique-named GL
+    kernels; writes profiles/pmc_traffic_{fp64,fp32}.json."""
+    import csv
+    import glob
+
+    def read(d, counter):
+        rows = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    key = (r["Dispatch_Id"], r["Kernel_Name"])
+                    rows[key] = rows.get(key, 0.0) + float(r["Counter_Value"])
+        return rows
+
+    fetch, write = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
+    nbytes = float(1 << 30)
+
+    def per_kernel(rows, match):
+        v = [val for (d, name), val in rows.items() if match(name)]
+        return sum(v) / len(v) if v else None
+
+    is_copy = lambda n: "elementwise" in n and "Mul" in n
+    copy_f = per_kernel(fetch, is_copy)
+    copy_w = per_kernel(write, is_copy)
+    copy_name = next((name for (d, name) in fetch if is_copy(name)), None)
+    sf = nbytes / (copy_f * 1024.0) if copy_f else 1.0
+    sw = nbytes / (copy_w * 1024.0) if copy_w else 1.0
+    prob = make_workload(size)
+    out = {}
+    for prec, tag, tname in ((pfdd_gl.PFDD_FP64, "fp64", "double"), (pfdd_gl.PFDD_FP32, "fp32", "float")):
+        s = solver_for(prob, prec)
+        kbytes, _ = s.traffic()
+        s.close()
+        algo = dict(zip(["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"], kbytes.tolist()))
+        res = {"copy_calibration": {"kernel": copy_name, "bytes_read": nbytes, "bytes_written": nbytes,
+                                    "fetch_kib": copy_f, "write_kib": copy_w,
+                                    "fetch_scale": sf, "write_scale": sw}, "kernels": {}}
+        for sub, role in KERNEL_ROLES.items():
+            m = lambda n, sub=sub: sub in n and f"<{tname}," in n
+            f, w = per_kernel(fetch, m), per_kernel(write, m)
+            if f is None or w is None:
+                continue
+            res["kernels"][role] = {"fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
+                                    "hbm_bytes_per_launch": f * 1024.0 * sf + w * 1024.0 * sw,
+                                    "algorithmic_bytes": algo[role]}
+        out[tag] = res
+        os.makedirs(out_dir, exist_ok=True)
+        path = os.path.join(out_dir, f"pmc_traffic_{tag}.json")
+        with open(path, "w") as fh:
+            json.dump(res, fh, indent=1)
+        print(f"wrote {path}")
+    return 0
 
 
 def cpu_baseline(prob, nsteps=3):
@@ -127,7 +213,16 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="workload for rocprofv3 --pmc runs (copy calibration + 3 steps)")
+    ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two rocprofv3 --pmc outputs into pmc_traffic_*.json")
+    ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles"))
     args = ap.parse_args()
+    if args.pmc_probe:
+        return pmc_probe(args.size)
+    if args.pmc_summarize:
+        return pmc_summarize(args.pmc_summarize[0], args.pmc_summarize[1], args.size, args.pmc_out)
     if args.gpus != 1:
         print(f"bench.py: this build runs the GL solver on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design note, not built.", file=sys.stderr)
@@ -140,9 +235,7 @@ def main():
         dt, r = time_steps(s, args.steps, args.warmup)
         xi_end = s.get_xi() if tag else None
         roof = roofline(s)
-        pmc = load_pmc_traffic(tag)
-        if pmc and pmc.get("kernel") == roof["kernel"]:
-            roof["traffic"] = pmc.get("hbm_bytes_per_launch")
+        roof["traffic"] = load_pmc_traffic(tag, roof["kernel"])
         out[tag] = {"sec": dt, "steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                     "roofline": roof, "e_total": r["e_total"], "xi_end": xi_end}
         s.close()
