@@ -40,9 +40,22 @@ def build_host():
     return out
 
 
+def build_driver():
+    """pfdd_hip: the deck driver (host C++) linked against the HIP library."""
+    out = os.path.join(HERE, "lib", "pfdd_hip")
+    srcs = [os.path.join(HERE, "host", f) for f in ("main.cpp", "pfdd_driver.cpp", "pfdd_host.cpp")]
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp"] + srcs +
+        ["-L" + os.path.join(HERE, "lib"), "-lpfdd_gl", "-Wl,-rpath,$ORIGIN",
+         "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(rocm, "lib")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def main():
     build_lib()
     build_host()
+    build_driver()
 
 
 if __name__ == "__main__":

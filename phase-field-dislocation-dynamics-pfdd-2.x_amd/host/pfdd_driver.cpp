@@ -1,0 +1,656 @@
+// pfdd_driver.cpp -- see pfdd_driver.hpp.
+#include "pfdd_driver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+namespace pfdd_host {
+
+namespace {
+
+[[noreturn]] void deck_error(const std::string &m) { throw DeckError{m}; }
+
+double wall()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+double num(const std::vector<std::string> &w, size_t i, const char *what)
+{
+    if (i >= w.size()) deck_error(std::string("Illegal ") + what + " command");
+    char *end = nullptr;
+    const double v = std::strtod(w[i].c_str(), &end);
+    if (end == w[i].c_str() || *end) deck_error(std::string("Illegal ") + what + " command: '" + w[i] + "'");
+    return v;
+}
+
+void check(int rc)
+{
+    if (rc != PFDD_OK) deck_error(std::string("GPU solver: ") + pfdd_gl_last_error());
+}
+
+} // namespace
+
+// ===================================================================== FFT_HIP
+FFT_HIP::~FFT_HIP()
+{
+    if (h_) pfdd_gl_destroy(h_);
+}
+
+void FFT_HIP::init()
+{
+    pfdd_gl_desc d{};
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.ns = slip_systems;
+    d.precision = precision;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) d.four[3 * r + c] = geo.four[r][c];
+    d.C44 = mat.C44;
+    d.C12 = mat.C12;
+    d.C11 = mat.C11;
+    d.S12 = mat.S12;
+    d.S44 = mat.S44;
+    d.dslip = mat.dslip;
+    d.xn = slip.xn.data();
+    d.xb = slip.xb.data();
+    d.cddt = CD * timestep;
+    std::vector<double> amp;
+    if (core_flag == 1) {
+        d.core_model = PFDD_CORE_PERFECT;
+    } else if (core_flag == 3) {
+        d.core_model = PFDD_CORE_SINE;
+    } else if (core_flag == 7) {
+        d.core_model = PFDD_CORE_MPEA;
+        const double mdb = mat.mu_s * mat.dslip * mat.b;
+        amp.resize(mat.A_mpea.size());
+        for (size_t q = 0; q < amp.size(); ++q) amp[q] = mat.A_mpea[q] / mdb;
+        d.amp = amp.data();
+    } else if (core_flag == 8) {
+        d.core_model = PFDD_CORE_SERIES;
+    } else if (core_flag == 0) {
+        d.core_model = PFDD_CORE_NONE;
+    } else {
+        deck_error("core_energy " + std::to_string(core_flag) + " is not available on the GPU path");
+    }
+    d.ncoef = ncoef;
+    d.coef = coef.empty() ? nullptr : coef.data();
+    bool any = false;
+    for (double v : xo) any |= v != 0.0;
+    d.xo = any ? xo.data() : nullptr;
+    d.device = 0;
+    d.flags = flags;
+    check(pfdd_gl_create(&d, &h_));
+    upload_xi();
+}
+
+void FFT_HIP::rotate_stress()
+{
+    pfdd_host::rotate_stress(geo, sigma, sigma_rot);
+}
+
+void FFT_HIP::resolSS_Schmid()
+{
+    tau.assign(slip_systems, 0.0);
+    resolss(slip, sigma_rot, mat.dslip, tau.data());
+    check(pfdd_gl_set_tau(h_, tau.data()));
+}
+
+int FFT_HIP::iterate_block(int n, double tol)
+{
+    check(pfdd_gl_step(h_, n, tol, &last));
+    xinorm = last.xinorm;
+    xiave = last.xiave;
+    E_core = last.e_core_ref;
+    host_current = false;
+    return last.steps_done;
+}
+
+void FFT_HIP::diagnostics()
+{
+    double e[9], s[9], sr[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) sr[3 * r + c] = sigma_rot[r][c];
+    check(pfdd_gl_diag(h_, sr, e, s));
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            ave_eps[r][c] = e[3 * r + c];
+            ave_sigma[r][c] = s[3 * r + c];
+        }
+}
+
+void FFT_HIP::sync_host()
+{
+    if (host_current) return;
+    check(pfdd_gl_get_xi(h_, xi.data()));
+    host_current = true;
+}
+
+void FFT_HIP::upload_xi()
+{
+    check(pfdd_gl_set_xi(h_, xi.data()));
+    host_current = true;
+}
+
+// ====================================================================== Output
+double Output::next_time(double tcurrent, double delta)
+{
+    // Output::next_time, logfreq 0 (output.cpp:393-400)
+    double tnew = std::ceil(tcurrent / delta) * delta;
+    if (tnew == tcurrent) tnew = tcurrent + delta;
+    return tnew;
+}
+
+void Output::stats_header()
+{
+    std::string s;
+    char buf[64];
+    snprintf(buf, sizeof buf, " %10s %10s", "Time", "Step");
+    s += buf;
+    snprintf(buf, sizeof buf, " %10s", "CPU");
+    s += buf;
+    if (diag_strain) {
+        const char *h[6] = {"e11", "e22", "e33", "e12", "e13", "e23"};
+        for (auto x : h) {
+            snprintf(buf, sizeof buf, " %8s", x);
+            s += buf;
+        }
+    }
+    if (diag_stress) {
+        const char *h[6] = {"s11", "s22", "s33", "s12", "s13", "s23"};
+        for (auto x : h) {
+            snprintf(buf, sizeof buf, " %8s", x);
+            s += buf;
+        }
+    }
+    fprintf(screen, "%s\n", s.c_str());
+    if (logfile) fprintf(logfile, "%s\n", s.c_str());
+}
+
+void Output::stats(App &app, FFT_HIP &fft, int timeflag)
+{
+    // Output::stats (output.cpp:325-355) with the app's time/steps columns and
+    // the %8g diagnostics columns (diag_strain.cpp:64-68, diag_stress.cpp:76-80)
+    std::string s;
+    char buf[128];
+    snprintf(buf, sizeof buf, " %10g %10d", app.time, app.steps);
+    s += buf;
+    snprintf(buf, sizeof buf, " %10.3g", timeflag ? loop_time : 0.0);
+    s += buf;
+    if (diag_strain) {
+        const double *e = &fft.ave_eps[0][0];
+        snprintf(buf, sizeof buf, " %8g %8g %8g %8g %8g %8g", e[0], e[4], e[8], e[1], e[2], e[5]);
+        s += buf;
+    }
+    if (diag_stress) {
+        const double *t = &fft.ave_sigma[0][0];
+        snprintf(buf, sizeof buf, " %8g %8g %8g %8g %8g %8g", t[0], t[4], t[8], t[1], t[2], t[5]);
+        s += buf;
+    }
+    fprintf(screen, "%s\n", s.c_str());
+    if (logfile) {
+        fprintf(logfile, "%s\n", s.c_str());
+        fflush(logfile);
+    }
+}
+
+void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
+{
+    // DumpText::write_header (dump_text.cpp:363-371) + per-site columns
+    fft.sync_host();
+    FILE *fp = fopen(d.file.c_str(), "a");
+    if (!fp) deck_error("Cannot open dump file " + d.file);
+    const long long N = (long long)fft.nx * fft.ny * fft.nz;
+    static int idump = 0;
+    std::string cols;
+    for (auto &f : d.fields) cols += f + " ";
+    if (!cols.empty()) cols.pop_back();
+    fprintf(fp, "ITEM: TIMESTEP\n%d %10g\nITEM: NUMBER OF ATOMS\n%lld\nITEM: BOX BOUNDS\n", idump++, time, N);
+    fprintf(fp, "%g %g\n%g %g\n%g %g\n", 0.0, app.scale[0] * fft.nx, 0.0, app.scale[1] * fft.ny, 0.0,
+            app.scale[2] * fft.nz);
+    fprintf(fp, "ITEM: ATOMS %s\n", cols.c_str());
+    for (long long r = 0; r < N; ++r) {
+        const int i = (int)(r / ((long long)fft.ny * fft.nz)), j = (int)((r / fft.nz) % fft.ny), k = (int)(r % fft.nz);
+        std::string line;
+        char buf[64];
+        for (auto &f : d.fields) {
+            double v = 0.0;
+            if (f == "x") v = i;
+            else if (f == "y") v = j;
+            else if (f == "z") v = k;
+            else if (f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0) {
+                const int a = std::atoi(f.c_str() + 3) - 1;
+                v = fft.xi[2 * (r + (long long)a * N) + (f[2] == 'r' ? 0 : 1)];
+            }
+            snprintf(buf, sizeof buf, "%g ", v);
+            line += buf;
+        }
+        if (!line.empty()) line.pop_back();
+        fprintf(fp, "%s\n", line.c_str());
+    }
+    fclose(fp);
+}
+
+double Output::setup(App &app, FFT_HIP &fft, double time)
+{
+    // Output::setup (output.cpp:79-126): first dumps, stats-driven diagnostics
+    // on the initial field, stats header and first line.
+    double dump_time = app.stoptime;
+    for (auto &d : dumps) {
+        write_dump(d, app, fft, time);
+        d.next_time = next_time(time, d.every);
+        dump_time = std::min(dump_time, d.next_time);
+    }
+    if (diag_strain || diag_stress) fft.diagnostics();
+    stats_header();
+    stats(app, fft, 0);
+    stats_time = app.stoptime;
+    if (stats_delta > 0.0) stats_time = next_time(time, stats_delta);
+    return std::min({app.stoptime, dump_time, stats_time});
+}
+
+double Output::compute(App &app, FFT_HIP &fft, double time, int done)
+{
+    // Output::compute (output.cpp:133-188)
+    double dump_time = app.stoptime;
+    for (auto &d : dumps) {
+        if (time >= d.next_time || done) {
+            write_dump(d, app, fft, time);
+            d.next_time = next_time(time, d.every);
+        }
+        dump_time = std::min(dump_time, d.next_time);
+    }
+    const bool sflag = time >= stats_time || done;
+    if (sflag) {
+        if (diag_strain || diag_stress) fft.diagnostics();
+        stats(app, fft, 1);
+        stats_time = app.stoptime;
+        if (stats_delta > 0.0) stats_time = next_time(time, stats_delta);
+    }
+    return std::min({app.stoptime, dump_time, stats_time});
+}
+
+// ========================================================================= App
+void App::init_app(FFT_HIP &fft)
+{
+    // init_app of the app style: set_slip (+ rotation), initial_sxtal, stopsteps
+    if (!slip_table(style, fft.slip)) deck_error("Unrecognized app style " + style);
+    rotate_slip(fft.slip, fft.geo);
+    fft.slip_systems = fft.slip.slip_systems;
+    fft.num_planes = fft.slip.num_planes;
+    fft.CD = CD >= 0 ? CD : fft.slip.CD;
+    fft.timestep = timestep >= 0 ? timestep : fft.slip.timestep;
+    fft.core_flag = core_flag >= 0 ? core_flag : fft.slip.default_core;
+    if (fft.core_flag == 1 || fft.core_flag == 3) {
+        fft.coef = core_coefficients(fft.mat, fft.core_flag);
+        fft.ncoef = (int)fft.coef.size();
+    }
+    if (fft.core_flag == 7 && fft.mat.A_mpea.empty()) deck_error("core_energy 7 needs 'material ... mpea FILE'");
+    Grid g{fft.nx, fft.ny, fft.nz, fft.slip_systems};
+    const long long n = (long long)g.S * g.N();
+    fft.xi.assign(2 * n, 0.0);
+    fft.xo.assign(n, 0.0);
+    if (init_style == "loops") {
+        init_random_loops(g, fft.slip, init_nloops, init_rmin, init_rmax, init_noise, seed, fft.xi.data());
+    } else if (init_style == "loop") {
+        init_sxtal_loop(g, fft.xi.data());
+    } else if (init_style == "frank_read") {
+        init_sxtal_frank_read(g, fft.xi.data(), fft.xo.data());
+    } else if (init_style == "edge") {
+        init_sxtal_edge(g, fft.xi.data());
+    } else if (init_style == "3SBCC") {
+        init_sxtal_3sbcc(g, fft.xi.data());
+    } else {
+        deck_error("Illegal setup command: initial " + init_style);
+    }
+    for (const Disc &dsc : discs) {
+        if (dsc.a < 0 || dsc.a >= g.S) deck_error("initial disc: slip system out of range");
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    const int p[3] = {i, j, k};
+                    if (p[dsc.axis] != dsc.plane) continue;
+                    const int u = dsc.axis == 0 ? j : i, v = dsc.axis == 2 ? j : k;
+                    if ((u - dsc.c0) * (u - dsc.c0) + (v - dsc.c1) * (v - dsc.c1) > dsc.r * dsc.r) continue;
+                    const long long idx = k + (long long)j * g.nz + (long long)i * g.ny * g.nz + (long long)dsc.a * g.N();
+                    fft.xi[2 * idx] += 1.0;
+                }
+    }
+    stopsteps = (int)std::floor(stoptime / fft.timestep);
+}
+
+// ===================================================================== SolveGL
+void SolveGL::iterate(App &app, FFT_HIP &fft, Output &out)
+{
+    // SolveGL::iterate (solve_GL.cpp:70-158).  Consecutive iterations that
+    // need no output are run as one GPU call; the device stops the call after
+    // the first iteration with xinorm < tol, so the break semantics match.
+    const double t0 = wall();
+    printf("Beginning Time Evolution\n");
+    for (int sstate = 0; sstate < app.stopsteps; ++sstate) {
+        fft.rotate_stress();
+        fft.resolSS_Schmid();
+        int it = 0;
+        while (it < max_iter) {
+            const double need = std::ceil(app.nextoutput - app.time); // first iteration that must output
+            int k = (int)std::min<double>(std::max(0.0, need), (double)(max_iter - 1 - it)) + 1;
+            k = std::max(1, k);
+            const int done = fft.iterate_block(k, tol);
+            ++gpu_calls;
+            it += done;
+            const int it_last = it - 1;
+            app.time += done - 1; // time of the last executed iteration
+            out.loop_time = wall() - t0;
+            const bool conv = fft.xinorm < tol;
+            if (conv || fft.ximin > 1 || it_last == max_iter - 1) {
+                printf("printing output it %d\n", it_last);
+                app.nextoutput = out.compute(app, fft, app.time, 1);
+            } else if (app.time >= app.nextoutput) {
+                app.nextoutput = out.compute(app, fft, app.time, 0);
+            }
+            app.time += 1;
+            app.steps += done;
+            if (conv) {
+                printf("Loading step %d converged after %d iterations\n", sstate, it_last);
+                break;
+            }
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) fft.sigma[i][j] = fft.sigma[i][j] + fft.deltasig[i][j];
+    }
+    out.loop_time = wall() - t0;
+}
+
+// ======================================================================= Input
+void Input::file(FILE *fp)
+{
+    // Input::file (input.cpp:117-190): '&' continues a line, '#' starts a comment
+    std::string acc;
+    char buf[4096];
+    while (fgets(buf, sizeof buf, fp)) {
+        std::string s = buf;
+        while (!s.empty() && (s.back() == '\n' || s.back() == '\r')) s.pop_back();
+        size_t amp = s.find_last_not_of(" \t");
+        if (amp != std::string::npos && s[amp] == '&') {
+            acc += s.substr(0, amp) + " ";
+            continue;
+        }
+        acc += s;
+        line(acc);
+        acc.clear();
+    }
+    if (!acc.empty()) line(acc);
+}
+
+void Input::line(const std::string &text)
+{
+    std::string s = text;
+    const size_t hash = s.find('#');
+    if (hash != std::string::npos) s = s.substr(0, hash);
+    std::istringstream is(s);
+    std::vector<std::string> w;
+    std::string t;
+    while (is >> t) w.push_back(t);
+    if (!w.empty()) command(w);
+}
+
+void Input::command(const std::vector<std::string> &w)
+{
+    const std::string &c = w[0];
+    if (c == "seed") {
+        app.seed = (int)num(w, 1, "seed");
+    } else if (c == "app_style") {
+        if (w.size() < 2) deck_error("Illegal app_style command");
+        app.style = w[1];
+        SlipTable probe;
+        if (!slip_table(app.style, probe)) deck_error("Unrecognized app style " + app.style);
+        for (size_t i = 2; i < w.size(); ++i) {
+            if (w[i] == "core_energy") app.core_flag = (int)num(w, ++i, "app_style");
+            else deck_error("Illegal app_style command: " + w[i]);
+        }
+    } else if (c == "fft_style") {
+        // FFT::FFT argument parsing (fft.cpp:96-153); style hip (or the
+        // reference's fftw_slab, accepted and run on the GPU)
+        if (w.size() < 2 || (w[1] != "hip" && w[1] != "fftw_slab")) deck_error("Invalid fft style");
+        for (size_t i = 2; i < w.size(); ++i) {
+            if (w[i] == "pbc") {
+                for (int q = 0; q < 3; ++q)
+                    if (num(w, ++i, "fft") != 1) deck_error("only periodic boxes are supported");
+            } else if (w[i] == "mode") {
+                fft.mode = (int)num(w, ++i, "fft");
+            } else if (w[i] == "primitive") {
+                double v[3][3];
+                for (int a = 0; a < 3; ++a) {
+                    if (++i >= w.size()) deck_error("Illegal fft command");
+                    const char ax = "xyz"[a];
+                    if (w[i] != std::string(1, ax)) deck_error("Illegal fft command: primitive");
+                    for (int q = 0; q < 3; ++q) v[a][q] = num(w, ++i, "fft");
+                }
+                fft.geo.set_primitive(v[0], v[1], v[2]);
+            } else if (w[i] == "precision") {
+                ++i;
+                if (i >= w.size()) deck_error("Illegal fft command");
+                if (w[i] == "fp64") fft.precision = PFDD_FP64;
+                else if (w[i] == "fp32") fft.precision = PFDD_FP32;
+                else deck_error("precision must be fp64 or fp32");
+            } else if (w[i] == "stored_B") {
+                fft.flags |= PFDD_GL_FLAG_STORED_B;
+            } else {
+                deck_error("Illegal fft_style command: " + w[i]);
+            }
+        }
+    } else if (c == "dimension") {
+        if ((int)num(w, 1, "dimension") != 3) deck_error("only dimension 3 is supported");
+    } else if (c == "lattice") {
+        // lattice style is irrelevant for the structured periodic FFT grid
+    } else if (c == "region") {
+        if (w.size() < 9 || w[2] != "block") deck_error("Illegal region command");
+        fft.nx = (int)(num(w, 4, "region") - num(w, 3, "region"));
+        fft.ny = (int)(num(w, 6, "region") - num(w, 5, "region"));
+        fft.nz = (int)(num(w, 8, "region") - num(w, 7, "region"));
+    } else if (c == "create_box") {
+        if (fft.nx <= 0) deck_error("Create_box region ID does not exist");
+        box = true;
+    } else if (c == "create_sites") {
+        if (!box) deck_error("Cannot create sites before simulation box is defined");
+        sites = true;
+    } else if (c == "material") {
+        // Material::Material (material.cpp:32-167)
+        Material &m = fft.mat;
+        for (size_t i = 1; i < w.size(); ++i) {
+            const std::string &k = w[i];
+            if (k == "preset") {
+                if (i + 1 >= w.size() || !m.preset(w[++i])) deck_error("Unknown material preset");
+            } else if (k == "mpea") {
+                if (i + 1 >= w.size()) deck_error("Illegal material command");
+                FILE *f = fopen(w[++i].c_str(), "r");
+                if (!f) deck_error("Cannot open input file " + w[i]);
+                int n1, n2, n3, ns;
+                if (fscanf(f, "%d %d %d %d", &n1, &n2, &n3, &ns) != 4) deck_error("bad MPEA file header");
+                m.A_mpea.assign((size_t)n1 * n2 * n3 * ns, 0.0);
+                int a, b, cc, s;
+                double v;
+                while (fscanf(f, "%d %d %d %d %lf", &a, &b, &cc, &s, &v) == 5)
+                    m.A_mpea[(size_t)a * n2 * n3 + (size_t)b * n3 + cc + (size_t)s * n1 * n2 * n3] = v * 1e-3;
+                fclose(f);
+            } else {
+                const double v = num(w, ++i, "material");
+                if (k == "a") m.a = v;
+                else if (k == "b") m.b = v;
+                else if (k == "mu") m.mu = v;
+                else if (k == "young") m.young = v;
+                else if (k == "isf") m.isf = v;
+                else if (k == "usf") m.usf = v;
+                else if (k == "An") m.An = v;
+                else if (k == "Cn") m.Cn = v;
+                else if (k == "c0") m.c0 = v;
+                else if (k == "c1") m.c1 = v;
+                else if (k == "c2") m.c2 = v;
+                else if (k == "c3") m.c3 = v;
+                else if (k == "c4") m.c4 = v;
+                else if (k == "a1") m.a1 = v;
+                else if (k == "a3") m.a3 = v;
+                else deck_error("Illegal material command: " + k);
+            }
+        }
+        if (m.b == 0.0 && m.a > 0.0) m.b = m.a / std::sqrt(2.0);
+        if (!(m.mu > 0) || !(m.young > 0)) deck_error("material needs mu and young (or a preset)");
+        m.derive();
+        have_material = true;
+    } else if (c == "solve_style") {
+        if (w.size() < 2 || w[1] != "GL") deck_error("Invalid solve style");
+        for (size_t i = 2; i < w.size(); ++i) {
+            if (w[i] == "max_iter") solve.max_iter = (int)num(w, ++i, "solve");
+            else if (w[i] == "tol") solve.tol = num(w, ++i, "solve");
+            else deck_error("Illegal solve command");
+        }
+    } else if (c == "diag_style") {
+        if (w.size() < 2) deck_error("Illegal diag_style command");
+        if (w[1] == "strain") output.diag_strain = true;
+        else if (w[1] == "stress") output.diag_stress = true;
+        else deck_error("Invalid diag_style " + w[1]);
+    } else if (c == "temperature") {
+        num(w, 1, "temperature");
+    } else if (c == "scale") {
+        // App::set_scale (app.cpp:383-444)
+        for (int q = 0; q < 3; ++q) app.scale[q] = num(w, 1 + q, "scale");
+        app.scale_set = true;
+    } else if (c == "sigma") {
+        // App::set_sigma (app.cpp:325-381): xx yy zz xy xz yz, units of mu
+        if (w.size() <= 7) deck_error("Illegal sigma command");
+        for (size_t i = 1; i < w.size();) {
+            double(*t)[3] = w[i] == "initial" ? app.sigma0 : w[i] == "delta" ? app.dsigma : nullptr;
+            if (!t) deck_error("Illegal sigma command");
+            double v[6];
+            for (int q = 0; q < 6; ++q) v[q] = num(w, i + 1 + q, "sigma");
+            t[0][0] = v[0];
+            t[1][1] = v[1];
+            t[2][2] = v[2];
+            t[0][1] = t[1][0] = v[3];
+            t[0][2] = t[2][0] = v[4];
+            t[1][2] = t[2][1] = v[5];
+            i += 7;
+        }
+    } else if (c == "stats") {
+        output.stats_delta = num(w, 1, "stats");
+    } else if (c == "dump") {
+        if (w.size() < 5 || w[2] != "text") deck_error("Illegal dump command");
+        DumpSpec d;
+        d.id = w[1];
+        d.every = num(w, 3, "dump");
+        d.file = w[4];
+        for (size_t i = 5; i < w.size(); ++i) {
+            const std::string &f = w[i];
+            const bool ok = f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0;
+            if (!ok) deck_error("dump field " + f + " is not available (fields: x y z xirN xiiN)");
+            d.fields.push_back(f);
+        }
+        output.dumps.push_back(d);
+    } else if (c == "run") {
+        run(w);
+    } else if (c == "initial") {
+        // extension: seeded initial fields
+        if (w.size() < 2) deck_error("Illegal initial command");
+        if (w[1] == "loops") {
+            app.init_style = "loops";
+            app.init_nloops = (int)num(w, 2, "initial");
+            app.init_rmin = num(w, 3, "initial");
+            app.init_rmax = num(w, 4, "initial");
+            app.init_noise = num(w, 5, "initial");
+        } else if (w[1] == "noise") {
+            app.init_style = "loops";
+            app.init_nloops = 0;
+            app.init_noise = num(w, 2, "initial");
+        } else if (w[1] == "disc") {
+            App::Disc d{(int)num(w, 2, "initial"), (int)num(w, 3, "initial"), (int)num(w, 4, "initial"),
+                        num(w, 5, "initial"), num(w, 6, "initial"), num(w, 7, "initial")};
+            if (d.axis < 0 || d.axis > 2) deck_error("initial disc: axis must be 0, 1 or 2");
+            app.discs.push_back(d);
+        } else if (w[1] == "sxtal") {
+            if (w.size() < 3) deck_error("Illegal initial command");
+            app.init_style = w[2];
+        } else {
+            deck_error("Illegal initial command");
+        }
+    } else if (c == "mobility") {
+        // extension: CD and timestep (hard-coded per app in the reference)
+        app.CD = num(w, 1, "mobility");
+        app.timestep = num(w, 2, "mobility");
+    } else if (c == "write_xi") {
+        if (w.size() < 2) deck_error("Illegal write_xi command");
+        write_xi_file = w[1];
+    } else {
+        deck_error("Unknown command: " + c);
+    }
+}
+
+void Input::print_setup()
+{
+    printf("grid %d %d %d slip_systems %d core_flag %d CD %.17g timestep %.17g\n", fft.nx, fft.ny, fft.nz,
+           fft.slip_systems, fft.core_flag, fft.CD, fft.timestep);
+    printf("four");
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) printf(" %.17g", fft.geo.four[r][c]);
+    printf("\nmaterial C44 %.17g C12 %.17g C11 %.17g S12 %.17g S44 %.17g mu_s %.17g b %.17g\n", fft.mat.C44,
+           fft.mat.C12, fft.mat.C11, fft.mat.S12, fft.mat.S44, fft.mat.mu_s, fft.mat.b);
+    printf("coef");
+    for (double v : fft.coef) printf(" %.17g", v);
+    printf("\n");
+    for (int a = 0; a < fft.slip_systems; ++a)
+        printf("slip %d xn %.17g %.17g %.17g xb %.17g %.17g %.17g\n", a, fft.slip.xn[3 * a], fft.slip.xn[3 * a + 1],
+               fft.slip.xn[3 * a + 2], fft.slip.xb[3 * a], fft.slip.xb[3 * a + 1], fft.slip.xb[3 * a + 2]);
+    rotate_stress(fft.geo, fft.sigma, fft.sigma_rot);
+    std::vector<double> tau(fft.slip_systems);
+    resolss(fft.slip, fft.sigma_rot, fft.mat.dslip, tau.data());
+    printf("tau");
+    for (double v : tau) printf(" %.17g", v);
+    printf("\nstopsteps %d max_iter %d tol %.17g\n", app.stopsteps, solve.max_iter, solve.tol);
+}
+
+void Input::run(const std::vector<std::string> &w)
+{
+    // App::run (app.cpp:127-205): init, setup, iterate, finish
+    if (!box || !sites) deck_error("Cannot run application until simulation box is defined");
+    if (!have_material) deck_error("material must be defined before run");
+    if (app.style.empty()) deck_error("app_style must be defined before run");
+    if (!app.scale_set) deck_error("scale must be set before run (it defines the reciprocal lattice)");
+    app.stoptime = app.time + num(w, 1, "run");
+    if (app.stoptime < app.time) deck_error("Illegal run command");
+    if (run_count++ == 0) {
+        fft.geo.set_scale(app.scale[0], app.scale[1], app.scale[2]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                fft.sigma[i][j] = app.sigma0[i][j];
+                fft.deltasig[i][j] = app.dsigma[i][j];
+            }
+        app.init_app(fft);
+        if (check_only) {
+            print_setup();
+            return;
+        }
+        fft.init();
+    } else {
+        app.stopsteps = (int)std::floor(app.stoptime / fft.timestep);
+        if (check_only) return;
+    }
+    printf("Setting up run ...\n");
+    fft.rotate_stress(); // FFTW_Slab::setup (fft_fftw_slab.cpp:111-119)
+    app.nextoutput = output.setup(app, fft, app.time);
+    solve.iterate(app, fft, output);
+    printf("Loop time of %g on %d procs\n", output.loop_time, 1);
+    if (!write_xi_file.empty()) {
+        fft.sync_host();
+        FILE *f = fopen(write_xi_file.c_str(), "wb");
+        if (!f) deck_error("Cannot open " + write_xi_file);
+        fwrite(fft.xi.data(), sizeof(double), fft.xi.size(), f);
+        fclose(f);
+    }
+}
+
+} // namespace pfdd_host

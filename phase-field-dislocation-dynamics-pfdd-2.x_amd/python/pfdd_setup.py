@@ -71,7 +71,7 @@ def material(preset=None, **over):
 def cu_like():
     """Benchmark material: mu 48 GPa, nu 0.34, a 0.3615 nm, b = a/sqrt2, isf/usf 41/158 mJ/m^2."""
     a = 0.3615e-9
-    return material(None, mu=48e9, young=2 * 48e9 * 1.34, a=a, b=a / np.sqrt(2.0), isf=41e-3, usf=158e-3)
+    return material(None, mu=48e9, young=128.64e9, a=a, b=a / np.sqrt(2.0), isf=41e-3, usf=158e-3)
 
 
 def slip(app, prim=None, scale=None):

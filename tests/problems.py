@@ -40,7 +40,7 @@ def basal3():
 
 # Cu-like isotropic material of the benchmark configurations
 MU, NU = 48e9, 0.34
-YOUNG = 2 * MU * (1 + NU)
+YOUNG = 128.64e9  # 2 mu (1 + nu)
 A_LAT = 0.3615e-9
 B_MAG = A_LAT / math.sqrt(2.0)
 ISF, USF = 41e-3, 158e-3
