@@ -661,6 +661,103 @@ __global__ void k_to_f64(const typename cvec<R>::type *__restrict__ src, double2
     if (q < n) dst[q] = make_double2((double)src[q].x, (double)src[q].y);
 }
 
+// ------------------------------------------------ output-step strain/stress
+// strain() (fft_fftw_slab.cpp:3370-3469) builds strain_ij(k) = sum_a F_a,ij(k)
+// xihat_a(k) with F_a,ij = f_i (G sigma_a f)_j (Fmatrix, :644-732; G with the
+// Voigt mu and nu), i.e. strain_ij = f_i v_j with v = G (Sig f), Sig =
+// sum_a xihat_a sigma_a.  Only the symmetric part enters data_eps and
+// stress(), so six components (xx,yy,zz,xy,xz,yz) are transformed back.
+struct DParams {
+    double sig6[MAXS * 6];
+    double epsym6[MAXS * 6]; // sym(eps_a): xx,yy,zz,(xy+yx)/2,(xz+zx)/2,(yz+zy)/2
+    double four[9];
+    double inv2mu, inv1mnu;
+    double C44, C12, invN;
+    double a_sym[6];    // sym(avepst) - sym(avepsts): the constant of stress()
+    double avepst6[6];  // sym(avepst), added to the strain
+    double sigmu[9];    // sigma_rot * mu
+    int nx, ny, nz, S;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(NTHR)
+    k_diag_comp(const typename cvec<R>::type *__restrict__ xh, typename cvec<R>::type *__restrict__ comp,
+                DParams P, long long ntot)
+{
+    typedef typename cvec<R>::type V;
+    const long long q = (long long)blockIdx.x * NTHR + threadIdx.x;
+    if (q >= ntot) return;
+    const int i = (int)(q % P.nx);
+    const long long col = q / P.nx;
+    const int j = (int)(col % P.ny), k = (int)(col / P.ny);
+    const double kx = frac_freq(i, P.nx), ky = frac_freq(j, P.ny), kz = frac_freq(k, P.nz);
+    double f[3];
+    for (int c = 0; c < 3; ++c) f[c] = kx * P.four[c] + ky * P.four[3 + c] + kz * P.four[6 + c];
+    const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    double Sr[6] = {0, 0, 0, 0, 0, 0}, Si[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < P.S; ++a) {
+        const V x = xh[(long long)a * ntot + q];
+        for (int c = 0; c < 6; ++c) {
+            Sr[c] += P.sig6[a * 6 + c] * (double)x.x;
+            Si[c] += P.sig6[a * 6 + c] * (double)x.y;
+        }
+    }
+    const double wr[3] = {Sr[0] * f[0] + Sr[3] * f[1] + Sr[4] * f[2], Sr[3] * f[0] + Sr[1] * f[1] + Sr[5] * f[2],
+                          Sr[4] * f[0] + Sr[5] * f[1] + Sr[2] * f[2]};
+    const double wi[3] = {Si[0] * f[0] + Si[3] * f[1] + Si[4] * f[2], Si[3] * f[0] + Si[1] * f[1] + Si[5] * f[2],
+                          Si[4] * f[0] + Si[5] * f[1] + Si[2] * f[2]};
+    const double Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
+    const double Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
+    double vr[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+    if (f2 > 0) {
+        const double if2 = 1.0 / f2, c4 = P.inv1mnu * if2 * if2;
+        for (int r = 0; r < 3; ++r) {
+            vr[r] = (2.0 * wr[r] * if2 - f[r] * Vr * c4) * P.inv2mu;
+            vi[r] = (2.0 * wi[r] * if2 - f[r] * Vi * c4) * P.inv2mu;
+        }
+    }
+    const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    for (int c = 0; c < 6; ++c) {
+        const double er = 0.5 * (f[ci[c]] * vr[cj[c]] + f[cj[c]] * vr[ci[c]]);
+        const double ei = 0.5 * (f[ci[c]] * vi[cj[c]] + f[cj[c]] * vi[ci[c]]);
+        comp[(long long)c * ntot + q] = mk<V>((R)er, (R)ei);
+    }
+}
+
+// Real space: data_eps (symmetric strain + sym(avepst)) and data_sigma =
+// C:(strain + avepst - eps_d(xi_post) - avepsts) + sigma_rot mu (stress(),
+// fft_fftw_slab.cpp:3474-3553), written [component (i + 3j)][cell].
+template <typename R>
+__global__ void __launch_bounds__(NTHR)
+    k_diag_real(const typename cvec<R>::type *__restrict__ comp, const typename cvec<R>::type *__restrict__ xi,
+                double2 *__restrict__ eps, double *__restrict__ sig, DParams P, long long ntot)
+{
+    const long long r = (long long)blockIdx.x * NTHR + threadIdx.x;
+    if (r >= ntot) return;
+    double er[6], ei[6], x[6];
+    for (int c = 0; c < 6; ++c) {
+        const auto v = comp[(long long)c * ntot + r];
+        er[c] = (double)v.x * P.invN;
+        ei[c] = (double)v.y * P.invN;
+        x[c] = er[c] + P.a_sym[c];
+    }
+    for (int a = 0; a < P.S; ++a) {
+        const double xr = (double)xi[(long long)a * ntot + r].x;
+        for (int c = 0; c < 6; ++c) x[c] -= P.epsym6[a * 6 + c] * xr;
+    }
+    const double tr = x[0] + x[1] + x[2];
+    const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    for (int c = 0; c < 6; ++c) {
+        const int ii = ci[c], jj = cj[c];
+        const double2 e = make_double2(er[c] + P.avepst6[c], ei[c]);
+        const double sv = 2.0 * P.C44 * x[c] + (ii == jj ? P.C12 * tr : 0.0);
+        eps[(long long)(ii + 3 * jj) * ntot + r] = e;
+        eps[(long long)(jj + 3 * ii) * ntot + r] = e;
+        sig[(long long)(ii + 3 * jj) * ntot + r] = sv + P.sigmu[3 * ii + jj];
+        sig[(long long)(jj + 3 * ii) * ntot + r] = sv + P.sigmu[3 * jj + ii];
+    }
+}
+
 // ------------------------------------------------------------------ dispatch
 #define PFDD_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
 
@@ -879,6 +976,9 @@ struct pfdd_gl {
     double C44 = 0, C12 = 0, C11 = 0, S12 = 0, S44 = 0, dslip = 0, mu = 0, xnu = 0;
     double four[9];
     bool stepped = false; // a step ran since the last set_xi (xi[1-cur] holds the pre-update field)
+    void *dw[2] = {nullptr, nullptr}; // output-step field workspace, max(S,6)*N complex each
+    double2 *deps = nullptr;          // 9*N strain field (complex)
+    double *dsig = nullptr;           // 9*N stress field (real)
 };
 
 namespace {
@@ -933,9 +1033,9 @@ struct Maps {
     long long n1, n2, n5;
 };
 
-Maps make_maps(const pfdd_gl *h)
+Maps make_maps(const pfdd_gl *h, long long S)
 {
-    const long long nx = h->nx, ny = h->ny, nz = h->nz, S = h->S;
+    const long long nx = h->nx, ny = h->ny, nz = h->nz;
     Maps m;
     // pass 1: lines (a,i | j), FFT along k
     m.p1i = {ny * nz, 0, nz, 1, ny, 1};
@@ -955,7 +1055,7 @@ Maps make_maps(const pfdd_gl *h)
 template <typename R>
 int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
 {
-    const Maps m = make_maps(h);
+    const Maps m = make_maps(h, h->S);
     void *xo = h->xi[h->cur];
     void *T2 = h->xi[1 - h->cur];
     hipStream_t st = h->stream;
@@ -1012,7 +1112,8 @@ void destroy_all(pfdd_gl *h)
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
-                    h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket};
+                    h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket, h->dw[0], h->dw[1],
+                    h->deps, h->dsig};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
@@ -1153,6 +1254,77 @@ int download_field_f64(pfdd_gl *h, const void *src, double *host)
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(tmp));
+    return PFDD_OK;
+}
+
+template <typename R>
+int diag_fields(pfdd_gl *h, const double *sigma_rot, const double *avepst, const double *avepsts,
+                double *data_eps, double *data_sigma)
+{
+    const long long S = h->S, N = h->N, SC = S > 6 ? S : 6;
+    const size_t wbytes = (size_t)SC * N * h->esz;
+    for (int q = 0; q < 2; ++q)
+        if (!h->dw[q]) HIPCHK(hipMalloc(&h->dw[q], wbytes));
+    if (!h->deps) HIPCHK(hipMalloc(&h->deps, sizeof(double2) * 9 * N));
+    if (!h->dsig) HIPCHK(hipMalloc(&h->dsig, sizeof(double) * 9 * N));
+    DParams P{};
+    const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    for (int a = 0; a < S; ++a)
+        for (int c = 0; c < 6; ++c) {
+            P.sig6[a * 6 + c] = h->G.sig6[a * 6 + c];
+            P.epsym6[a * 6 + c] = 0.5 * (h->eps[a * 9 + 3 * ci[c] + cj[c]] + h->eps[a * 9 + 3 * cj[c] + ci[c]]);
+        }
+    for (int q = 0; q < 9; ++q) {
+        P.four[q] = h->four[q];
+        P.sigmu[q] = sigma_rot[q] * h->mu;
+    }
+    for (int c = 0; c < 6; ++c) {
+        const int i = ci[c], j = cj[c];
+        P.avepst6[c] = 0.5 * (avepst[3 * i + j] + avepst[3 * j + i]);
+        P.a_sym[c] = P.avepst6[c] - 0.5 * (avepsts[3 * i + j] + avepsts[3 * j + i]);
+    }
+    P.inv2mu = 1.0 / (2.0 * h->mu);
+    P.inv1mnu = 1.0 / (1.0 - h->xnu);
+    P.C44 = h->C44;
+    P.C12 = h->C12;
+    P.invN = 1.0 / (double)N;
+    P.nx = h->nx;
+    P.ny = h->ny;
+    P.nz = h->nz;
+    P.S = h->S;
+    // strain() reads the k-space field of the pre-update xi (quirk Q2)
+    const void *pre = h->stepped ? h->xi[1 - h->cur] : h->xi[h->cur];
+    hipStream_t st = h->stream;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, h->done, h->steps);
+    HIPCHK(hipGetLastError());
+    const Maps ms = make_maps(h, S), m6 = make_maps(h, 6);
+    const long long ncol = (long long)h->ny * h->nz;
+    const LineMap xs = {h->nx, 0, 0, 1, 1, 1}; // x lines of the [.][k][j][i] layout
+    HIPCHK((launch_pass<R, false>(h->nz, pre, h->dw[0], ms.p1i, ms.p1o, ms.n1, h->tw[2], h->done, st)));
+    HIPCHK((launch_pass<R, false>(h->ny, h->dw[0], h->dw[1], ms.p2i, ms.p2o, ms.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<R, false>(h->nx, h->dw[1], h->dw[0], xs, xs, S * ncol, h->tw[0], h->done, st)));
+    const unsigned nb = (unsigned)((N + NTHR - 1) / NTHR);
+    typedef typename cvec<R>::type V;
+    hipLaunchKernelGGL((k_diag_comp<R>), dim3(nb), dim3(NTHR), 0, st, (const V *)h->dw[0], (V *)h->dw[1], P, N);
+    HIPCHK(hipGetLastError());
+    HIPCHK((launch_pass<R, true>(h->nx, h->dw[1], h->dw[0], xs, xs, 6 * ncol, h->tw[0], h->done, st)));
+    HIPCHK((launch_pass<R, true>(h->ny, h->dw[0], h->dw[1], m6.p2o, m6.p2i, m6.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<R, true>(h->nz, h->dw[1], h->dw[0], m6.p5i, m6.p5o, m6.n5, h->tw[2], h->done, st)));
+    hipLaunchKernelGGL((k_diag_real<R>), dim3(nb), dim3(NTHR), 0, st, (const V *)h->dw[0],
+                       (const V *)h->xi[h->cur], h->deps, h->dsig, P, N);
+    HIPCHK(hipGetLastError());
+    if (data_eps) HIPCHK(hipMemcpyAsync(data_eps, h->deps, sizeof(double2) * 9 * N, hipMemcpyDeviceToHost, st));
+    std::vector<double> sg;
+    if (data_sigma) {
+        sg.resize((size_t)9 * N);
+        HIPCHK(hipMemcpyAsync(sg.data(), h->dsig, sizeof(double) * 9 * N, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (data_sigma)
+        for (long long q = 0; q < 9 * N; ++q) {
+            data_sigma[2 * q] = sg[q];
+            data_sigma[2 * q + 1] = 0.0;
+        }
     return PFDD_OK;
 }
 
@@ -1438,8 +1610,6 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], 
     int rc;
     if ((rc = check_handle(h))) return rc;
     if (!sigma_rot || !ave_eps || !ave_sigma) return fail(PFDD_ERR_ARG, "null argument");
-    if (data_eps || data_sigma)
-        return fail(PFDD_ERR_UNSUPPORTED, "field output of the diagnostics is not available in this build");
     std::vector<double> sums(h->S);
     if ((rc = pfdd_gl_xi_sums(h, sums.data()))) return rc;
     const double nsize = (double)h->N;
@@ -1476,6 +1646,12 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], 
                     s += Cijkl(i, j, m, l) * (avepst[3 * m + l] - ave_epsd[3 * m + l] - avepsts[3 * m + l]);
             ave_sigma[3 * i + j] = s + sigma_rot[3 * i + j] * h->mu;
         }
+    if (data_eps || data_sigma) {
+        if ((rc = set_device(h))) return rc;
+        rc = h->prec == PFDD_FP64 ? diag_fields<double>(h, sigma_rot, avepst, avepsts, data_eps, data_sigma)
+                                  : diag_fields<float>(h, sigma_rot, avepst, avepsts, data_eps, data_sigma);
+        if (rc) return rc;
+    }
     return PFDD_OK;
 }
 

@@ -117,7 +117,14 @@ void FFT_HIP::diagnostics()
     double e[9], s[9], sr[9];
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) sr[3 * r + c] = sigma_rot[r][c];
-    check(pfdd_gl_diag(h_, sr, e, s));
+    if (want_fields) {
+        const size_t n = (size_t)18 * nx * ny * nz;
+        if (data_eps.size() != n) data_eps.assign(n, 0.0);
+        if (data_sigma.size() != n) data_sigma.assign(n, 0.0);
+        check(pfdd_gl_diag_fields(h_, sr, e, s, data_eps.data(), data_sigma.data()));
+    } else {
+        check(pfdd_gl_diag(h_, sr, e, s));
+    }
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) {
             ave_eps[r][c] = e[3 * r + c];
@@ -227,6 +234,13 @@ void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
             else if (f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0) {
                 const int a = std::atoi(f.c_str() + 3) - 1;
                 v = fft.xi[2 * (r + (long long)a * N) + (f[2] == 'r' ? 0 : 1)];
+            } else if (f.size() == 3 && (f[0] == 'p' || f[0] == 'e')) {
+                // AppFCC::compute_stress / compute_strain (app_fcc.cpp:438-502): component
+                // (i, j) of the last computed data_sigma / data_eps.  As in the reference,
+                // dumps are written before the diagnostics of the same output step.
+                const int i = f[1] - 'x', j = f[2] - 'x';
+                const std::vector<double> &src = f[0] == 'p' ? fft.data_sigma : fft.data_eps;
+                v = src.empty() ? 0.0 : src[2 * (r + (long long)i * N + (long long)j * 3 * N)];
             }
             snprintf(buf, sizeof buf, "%g ", v);
             line += buf;
@@ -547,8 +561,13 @@ void Input::command(const std::vector<std::string> &w)
         d.file = w[4];
         for (size_t i = 5; i < w.size(); ++i) {
             const std::string &f = w[i];
-            const bool ok = f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0;
-            if (!ok) deck_error("dump field " + f + " is not available (fields: x y z xirN xiiN)");
+            static const char *tens[12] = {"pxx", "pyy", "pzz", "pxy", "pxz", "pyz",
+                                           "exx", "eyy", "ezz", "exy", "exz", "eyz"};
+            bool t = false;
+            for (auto q : tens) t |= f == q;
+            const bool ok = t || f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0;
+            if (!ok) deck_error("dump field " + f + " is not available (x y z xirN xiiN pxx..pyz exx..eyz)");
+            if (t) fft.want_fields = true;
             d.fields.push_back(f);
         }
         output.dumps.push_back(d);
@@ -630,6 +649,12 @@ void Input::run(const std::vector<std::string> &w)
                 fft.deltasig[i][j] = app.dsigma[i][j];
             }
         app.init_app(fft);
+        for (auto &d : output.dumps)
+            for (auto &f : d.fields)
+                if (f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0) {
+                    const int a = std::atoi(f.c_str() + 3);
+                    if (a < 1 || a > fft.slip_systems) deck_error("dump field " + f + ": no such slip system");
+                }
         if (check_only) {
             print_setup();
             return;

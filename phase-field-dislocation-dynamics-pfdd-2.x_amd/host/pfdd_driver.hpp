@@ -36,6 +36,8 @@ class FFT_HIP {
     double sigma[3][3] = {}, deltasig[3][3] = {}, sigma_rot[3][3] = {};
     std::vector<double> tau;
     double ave_eps[3][3] = {}, ave_sigma[3][3] = {};
+    std::vector<double> data_eps, data_sigma; // fft->data_eps / data_sigma (2*9*N), when dumped
+    bool want_fields = false;
     Geometry geo;
     Material mat;
     SlipTable slip;
@@ -65,7 +67,7 @@ class FFT_HIP {
     // n iterations in one call; stops on the device after the first one with
     // xinorm < tol.  Returns the number executed.
     int iterate_block(int n, double tol);
-    void diagnostics();   // strain() + total_average_strain() + stress() averages
+    void diagnostics();   // strain() + total_average_strain() + stress(): averages (+ fields)
     void sync_host();     // xi device -> host mirror
     void upload_xi();     // host mirror -> device
     pfdd_gl *handle() const { return h_; }

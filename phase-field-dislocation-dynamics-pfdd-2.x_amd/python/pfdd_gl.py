@@ -218,6 +218,21 @@ class GLSolver:
         _check(self._lib.pfdd_gl_diag(self._h, _ptr(_f64(sigma_rot).reshape(-1)), _ptr(ae), _ptr(asg)))
         return ae.reshape(3, 3), asg.reshape(3, 3)
 
+    def diag_fields(self, sigma_rot):
+        """Averages plus the output-step fields data_eps (complex) and
+        data_sigma (real part; the reference's imaginary part is 0), each of
+        shape (3 [j], 3 [i], nx, ny, nz) as the reference indexes them
+        (r + i*N + j*3N)."""
+        ae, asg = np.zeros(9), np.zeros(9)
+        N = int(np.prod(self.shape))
+        de = np.zeros(2 * 9 * N)
+        ds = np.zeros(2 * 9 * N)
+        _check(self._lib.pfdd_gl_diag_fields(self._h, _ptr(_f64(sigma_rot).reshape(-1)), _ptr(ae),
+                                             _ptr(asg), _ptr(de), _ptr(ds)))
+        return (ae.reshape(3, 3), asg.reshape(3, 3),
+                de.view(np.complex128).reshape((3, 3) + self.shape),
+                ds.view(np.complex128).reshape((3, 3) + self.shape))
+
     def profile(self, nsteps=10):
         ms = np.zeros(7)
         nk = ctypes.c_int()

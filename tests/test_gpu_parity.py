@@ -163,6 +163,33 @@ def test_diag_averages_match_oracle():
         g.close()
 
 
+@pytest.mark.parametrize("precision,steps,skew", [(0, 0, False), (0, 4, False), (0, 3, True), (1, 4, False)])
+def test_diag_fields_match_oracle(precision, steps, skew):
+    _say(f"case: 16x16x32 x 3 output-step strain/stress fields, precision {precision}, {steps} steps, skew {skew}")
+    prim = ((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1)) if skew else ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+    case = P.make_case(shape=(16, 16, 32), noise=1e-2, sigma_mu=0.004, prim_vectors=prim,
+                       loops=[(0, 2, 16, 8, 8, 5)])
+    m = case["mat"]
+    BB, eps, mod = P.oracle_B(case)
+    st = P.oracle_state(case, BB)
+    g = P.gpu_solver(case, precision)
+    try:
+        for _ in range(steps):
+            st.step(case["tau"])
+            g.step(1)
+        FF = oracle.fmatrix(case["shape"], m["C44"], m["C12"], mod["mu"], mod["xnu"], eps,
+                            case["geo"]["four"])
+        pre = st.data_fftw if steps else np.stack([oracle.fft3d(x, -1) for x in st.xi])
+        d = oracle.diagnostics(pre, st.xi, FF, eps, m, mod["mu"], case["sigma_rot"], fields=True)
+        ae, asg, de, ds = g.diag_fields(case["sigma_rot"])
+        tol = 1e-9 if precision == 0 else 1e-5
+        assert relerr(de, d["data_eps"]) < tol, relerr(de, d["data_eps"])
+        assert relerr(ds, d["data_sigma"]) < tol, relerr(ds, d["data_sigma"])
+        assert np.max(np.abs(ae - d["ave_eps"])) < tol * np.max(np.abs(d["ave_eps"]))
+    finally:
+        g.close()
+
+
 def test_nonfinite_is_reported():
     _say("case: NaN in the initial field -> PFDD_ERR_NONFINITE")
     import pfdd_gl
