@@ -5,7 +5,7 @@
 // The transform is FFTW 2's (fft_fftw_slab.cpp:1962,2021): unnormalised,
 // forward sign -1, backward sign +1.
 //
-// Tile layout in LDS: element (line l, position p) at lds[p * (NL + 1) + l].
+// Tile layout in LDS: element (line l, position p) at index p * (NL + 1) + l.
 // Butterflies are mapped line-fastest across threads, so a wave touches
 // consecutive LDS addresses; the +1 pad keeps position-fastest accesses (the
 // coalesced load/store of contiguous lines) free of bank conflicts.
@@ -39,6 +39,37 @@ template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
 {
     return INV ? mk<V>(-a.y, a.x) : mk<V>(a.y, -a.x);
 }
+
+
+// ------------------------------------------------------------ LDS tiles
+// Storage of a tile of complex values in LDS.  float2 values are kept
+// interleaved (8-byte accesses); double2 values are kept as two planes of
+// doubles (re, im) so that every LDS access is 8 bytes wide: 16-byte
+// accesses to the same tile patterns measured heavy bank conflicts.
+template <typename V> struct Tile;
+template <> struct Tile<float2> {
+    float2 *p;
+    __device__ __forceinline__ float2 ld(int i) const { return p[i]; }
+    __device__ __forceinline__ void st(int i, float2 v) const { p[i] = v; }
+};
+template <> struct Tile<double2> {
+    double *re, *im;
+    __device__ __forceinline__ double2 ld(int i) const { return make_double2(re[i], im[i]); }
+    __device__ __forceinline__ void st(int i, double2 v) const
+    {
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+};
+__device__ __forceinline__ Tile<float2> make_tile(float *mem, int) { return Tile<float2>{(float2 *)mem}; }
+__device__ __forceinline__ Tile<double2> make_tile(double *mem, int count)
+{
+    return Tile<double2>{mem, mem + count};
+}
+// Declares `name` (a Tile<V>) over COUNT complex values of static LDS.
+#define PFDD_TILE(V, name, COUNT)                                                \
+    __shared__ __align__(16) decltype(V{}.x) name##_mem[2 * (COUNT)];              \
+    const Tile<V> name = make_tile(name##_mem, (COUNT))
 
 // ---------------------------------------------------------------- butterflies
 template <bool INV, typename V> __device__ __forceinline__ void dft2(V *v)
@@ -136,7 +167,7 @@ template <typename R, int N> constexpr int tile_lines()
 //   output (line l, (j/NS)*NS*RAD + (j mod NS) + r*NS)
 // tw[m] = exp(-2 pi i m / N), m < N.
 template <int N, int RAD, int NS, int NL, int NTHR, bool INV, typename V>
-__device__ __forceinline__ void stockham_stage(V *lds, const V *__restrict__ tw, int tid)
+__device__ __forceinline__ void stockham_stage(const Tile<V> lds, const V *__restrict__ tw, int tid)
 {
     constexpr int LDP = NL + 1;
     constexpr int NBL = N / RAD;
@@ -149,7 +180,7 @@ __device__ __forceinline__ void stockham_stage(V *lds, const V *__restrict__ tw,
         if ((NBF % NTHR == 0) || bf < NBF) {
             const int l = bf % NL, j = bf / NL;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) v[q][r] = lds[(j + r * NBL) * LDP + l];
+            for (int r = 0; r < RAD; ++r) v[q][r] = lds.ld((j + r * NBL) * LDP + l);
         }
     }
     __syncthreads();
@@ -167,14 +198,14 @@ __device__ __forceinline__ void stockham_stage(V *lds, const V *__restrict__ tw,
             dft<RAD, INV>(v[q]);
             const int idxD = (j / NS) * NS * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) lds[(idxD + r * NS) * LDP + l] = v[q][r];
+            for (int r = 0; r < RAD; ++r) lds.st((idxD + r * NS) * LDP + l, v[q][r]);
         }
     }
     __syncthreads();
 }
 
 template <int N, int NS, int NL, int NTHR, bool INV, typename V>
-__device__ __forceinline__ void fft_stages(V *lds, const V *__restrict__ tw, int tid)
+__device__ __forceinline__ void fft_stages(const Tile<V> lds, const V *__restrict__ tw, int tid)
 {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix(N / NS);
@@ -205,7 +236,7 @@ __device__ __forceinline__ long long line_base(const LineMap &m, long long L)
 // before its first LDS write.  16-byte elements: a plain strided loop, which
 // measured faster than any batched form on gfx950.
 template <int N, int NL, int NTHR, typename V>
-__device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
+__device__ __forceinline__ void tile_load(const Tile<V> lds, const V *__restrict__ src,
                                           const long long *base, long long SP, int tid)
 {
     constexpr int LDP = NL + 1, TOT = N * NL, PER = (TOT + NTHR - 1) / NTHR;
@@ -215,14 +246,14 @@ __device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
             for (int e = tid; e < TOT; e += NTHR) {
                 const int l = e / N, p = e % N;
                 const long long b = base[l];
-                if (b >= 0) lds[p * LDP + l] = src[b + p];
+                if (b >= 0) lds.st(p * LDP + l, src[b + p]);
             }
         } else {
 #pragma unroll 4
             for (int e = tid; e < TOT; e += NTHR) {
                 const int l = e % NL, p = e / NL;
                 const long long b = base[l];
-                if (b >= 0) lds[p * LDP + l] = src[b + p * SP];
+                if (b >= 0) lds.st(p * LDP + l, src[b + p * SP]);
             }
         }
     } else {
@@ -237,7 +268,7 @@ __device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
 #pragma unroll
             for (int t = 0; t < PER; ++t) {
                 const int e = tid + t * NTHR;
-                if (TOT % NTHR == 0 || e < TOT) lds[(e % N) * LDP + e / N] = r[t];
+                if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[t]);
             }
         } else {
 #pragma unroll
@@ -249,14 +280,14 @@ __device__ __forceinline__ void tile_load(V *lds, const V *__restrict__ src,
 #pragma unroll
             for (int t = 0; t < PER; ++t) {
                 const int e = tid + t * NTHR;
-                if (TOT % NTHR == 0 || e < TOT) lds[(e / NL) * LDP + e % NL] = r[t];
+                if (TOT % NTHR == 0 || e < TOT) lds.st((e / NL) * LDP + e % NL, r[t]);
             }
         }
     }
 }
 
 template <int N, int NL, int NTHR, typename V>
-__device__ __forceinline__ void tile_store(V *__restrict__ dst, const V *lds,
+__device__ __forceinline__ void tile_store(V *__restrict__ dst, const Tile<V> lds,
                                            const long long *base, long long SP, int tid)
 {
     constexpr int LDP = NL + 1, TOT = N * NL;
@@ -265,14 +296,14 @@ __device__ __forceinline__ void tile_store(V *__restrict__ dst, const V *lds,
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e / N, p = e % N;
             const long long b = base[l];
-            if (b >= 0) dst[b + p] = lds[p * LDP + l];
+            if (b >= 0) dst[b + p] = lds.ld(p * LDP + l);
         }
     } else {
 #pragma unroll 4
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e % NL, p = e / NL;
             const long long b = base[l];
-            if (b >= 0) dst[b + p * SP] = lds[p * LDP + l];
+            if (b >= 0) dst[b + p * SP] = lds.ld(p * LDP + l);
         }
     }
 }

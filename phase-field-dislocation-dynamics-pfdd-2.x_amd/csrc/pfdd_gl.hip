@@ -94,6 +94,7 @@ struct GParams {
     double eps6[MAXS * 6];
     double four[9];
     double inv2C44, inv1mnu, invmu;
+    double C44x2, C12; // isotropic stiffness: C:Q = 2 C44 Q + C12 tr(Q) I
     int nx, ny, nz;
 };
 
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(NTHR)
     typedef typename cvec<R>::type V;
     if (*done) return;
     constexpr int NL = tile_lines<R, N>(), LDP = NL + 1;
-    __shared__ V lds[N * LDP];
+    PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     const int tid = threadIdx.x;
     const long long L0 = (long long)blockIdx.x * NL;
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(NTHR)
 // Load the S lines of NP consecutive columns (T2[a][c0+q][0..N)) into the
 // tile, lines ordered (a, q); column-contiguous runs of N elements.
 template <int N, int NL, int NP, int NTHR, typename V>
-__device__ __forceinline__ void tile_load_cols(V *lds, const V *__restrict__ T2, long long c0, long long ncol,
+__device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__restrict__ T2, long long c0, long long ncol,
                                                long long ntot, int tid)
 {
     constexpr int LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void tile_load_cols(V *lds, const V *__restrict__ T2,
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e / N, p = e % N;
             const int a = l / NP, q = l % NP;
-            if (c0 + q < ncol) lds[p * LDP + l] = T2[(long long)a * ntot + (c0 + q) * N + p];
+            if (c0 + q < ncol) lds.st(p * LDP + l, T2[(long long)a * ntot + (c0 + q) * N + p]);
         }
     } else {
         V r[PER];
@@ -199,7 +200,7 @@ __device__ __forceinline__ void tile_load_cols(V *lds, const V *__restrict__ T2,
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
             const int e = tid + t * NTHR;
-            if (TOT % NTHR == 0 || e < TOT) lds[(e % N) * LDP + e / N] = r[t];
+            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[t]);
         }
     }
 }
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(NTHR)
     constexpr int NP = mid_pairs<R, N, S>();
     constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
     constexpr int NPAIR = S * (S + 1) / 2;
-    __shared__ V lds[N * LDP];
+    PFDD_TILE(V, lds, N * LDP);
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
     const long long c0 = (long long)blockIdx.x * NP;
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) lds[p * LDP + l] = T2[(long long)a * ntot + (c0 + q) * N + p];
+        if (c0 + q < ncol) lds.st(p * LDP + l, T2[(long long)a * ntot + (c0 + q) * N + p]);
     }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
@@ -244,7 +245,7 @@ __global__ void __launch_bounds__(NTHR)
         const long long kp = (c0 + q) * N + p;
         V x[S];
 #pragma unroll
-        for (int b = 0; b < S; ++b) x[b] = lds[p * LDP + b * NP + q];
+        for (int b = 0; b < S; ++b) x[b] = lds.ld(p * LDP + b * NP + q);
         R g[S][2];
 #pragma unroll
         for (int a = 0; a < S; ++a) g[a][0] = g[a][1] = (R)0;
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(NTHR)
 #pragma unroll
         for (int a = 0; a < S; ++a) {
             el += (double)x[a].x * (double)g[a][0] + (double)x[a].y * (double)g[a][1];
-            lds[p * LDP + a * NP + q] = mk<V>(g[a][0], g[a][1]);
+            lds.st(p * LDP + a * NP + q, mk<V>(g[a][0], g[a][1]));
         }
     }
     __syncthreads();
@@ -275,7 +276,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds[p * LDP + l];
+        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
     }
     double v[1] = {el};
     block_sum<1, NTHR>(v, red, tid);
@@ -284,11 +285,15 @@ __global__ void __launch_bounds__(NTHR)
 
 
 // x pass with B(k) evaluated on the fly in FP64 instead of streamed from HBM.
-// With the complex stress of the transformed eigenstrain Sig = sum_b xihat_b
-// sigma_b, w = Sig f and u_a = sigma_a f,
-//   ghat_a = (1/mu) [ Sig:eps_a - (2 (u_a.w)/f^2 - (u_a.f)(f.w)/((1-nu) f^4)) / (2 C44) ]
-// which is sum_b B_ab(k) xihat_b for the kernel of Bmatrix (fft_fftw_slab.cpp:
-// 577-634) applied as internal_energy does (:1670-1695); ghat = 0 at f = 0
+// Bmatrix's kernel (fft_fftw_slab.cpp:577-634) is mu B_ab = eps_a:A:eps_b with
+// A = C - (C f) G (f C), G = (2 I/f^2 - f f/((1-nu) f^4))/(2 C44).  Applied as
+// internal_energy does (:1670-1695) to all systems at once:
+//   Sig = sum_b xihat_b sigma_b        (sigma_b = C:eps_b)
+//   w = Sig f,  V = f.w
+//   Q = [2 sym(w f)/f^2 - V f f/((1-nu) f^4)] / (2 C44)
+//   P = Sig - C:Q = Sig - 2 C44 Q - C12 tr(Q) I
+//   ghat_a = eps_a : P / mu
+// (sigma_a:Q = eps_a:(C:Q) for the isotropic C), and ghat = 0 at f = 0
 // (B(0) = 0).  TPP threads share one k-point (wave-uniform split of the
 // systems), each producing S/TPP of the outputs.
 template <typename R, int N, int S> constexpr int otf_tpp()
@@ -299,51 +304,62 @@ template <typename R, int N, int S> constexpr int otf_tpp()
 
 // Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im).
 template <int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
-__device__ __forceinline__ void otf_point(const V *lds, int p, int q, const double f[3], double f2,
+__device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const double f[3], double f2,
                                           const double *__restrict__ cst, const GParams &G, Sink sink)
 {
     // compiler-only barriers keep the (LDS-broadcast) constants from being
     // hoisted and held in registers across points and systems
     __asm__ volatile("" ::: "memory");
-    double Sr[6] = {0, 0, 0, 0, 0, 0}, Si[6] = {0, 0, 0, 0, 0, 0};
+    double Pr[6] = {0, 0, 0, 0, 0, 0}, Pi[6] = {0, 0, 0, 0, 0, 0}; // Sig, then P
 #pragma unroll
     for (int b = 0; b < S; ++b) {
-        const V v = lds[p * LDP + b * NP + q];
+        const V v = lds.ld(p * LDP + b * NP + q);
         const double xr = (double)v.x, xi = (double)v.y;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            Sr[c] += cst[b * 12 + c] * xr;
-            Si[c] += cst[b * 12 + c] * xi;
+            Pr[c] += cst[b * 12 + c] * xr;
+            Pi[c] += cst[b * 12 + c] * xi;
         }
     }
-    // w = Sig f (symmetric Sig: xx,yy,zz,xy,xz,yz)
-    const double wr[3] = {Sr[0] * f[0] + Sr[3] * f[1] + Sr[4] * f[2], Sr[3] * f[0] + Sr[1] * f[1] + Sr[5] * f[2],
-                          Sr[4] * f[0] + Sr[5] * f[1] + Sr[2] * f[2]};
-    const double wi[3] = {Si[0] * f[0] + Si[3] * f[1] + Si[4] * f[2], Si[3] * f[0] + Si[1] * f[1] + Si[5] * f[2],
-                          Si[4] * f[0] + Si[5] * f[1] + Si[2] * f[2]};
-    const double Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
-    const double Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
-    const double if2 = f2 > 0 ? 1.0 / f2 : 0.0;
-    const double c2 = 2.0 * if2 * G.inv2C44;
-    const double c4 = G.inv1mnu * if2 * if2 * G.inv2C44;
+    if (f2 > 0) {
+        // symmetric components xx, yy, zz, xy, xz, yz
+        const double wr[3] = {Pr[0] * f[0] + Pr[3] * f[1] + Pr[4] * f[2], Pr[3] * f[0] + Pr[1] * f[1] + Pr[5] * f[2],
+                              Pr[4] * f[0] + Pr[5] * f[1] + Pr[2] * f[2]};
+        const double wi[3] = {Pi[0] * f[0] + Pi[3] * f[1] + Pi[4] * f[2], Pi[3] * f[0] + Pi[1] * f[1] + Pi[5] * f[2],
+                              Pi[4] * f[0] + Pi[5] * f[1] + Pi[2] * f[2]};
+        const double Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
+        const double Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
+        const double if2 = 1.0 / f2;
+        const double c2 = if2 * G.inv2C44;                 // multiplies (w_i f_j + w_j f_i)
+        const double c4 = G.inv1mnu * if2 * if2 * G.inv2C44; // multiplies V f_i f_j
+        const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+        double Qr[6], Qi[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double ff = f[ci[c]] * f[cj[c]];
+            Qr[c] = c2 * (wr[ci[c]] * f[cj[c]] + wr[cj[c]] * f[ci[c]]) - c4 * Vr * ff;
+            Qi[c] = c2 * (wi[ci[c]] * f[cj[c]] + wi[cj[c]] * f[ci[c]]) - c4 * Vi * ff;
+        }
+        const double tr_r = G.C12 * (Qr[0] + Qr[1] + Qr[2]), tr_i = G.C12 * (Qi[0] + Qi[1] + Qi[2]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            Pr[c] -= G.C44x2 * Qr[c] + (c < 3 ? tr_r : 0.0);
+            Pi[c] -= G.C44x2 * Qi[c] + (c < 3 ? tr_i : 0.0);
+        }
+    }
 #pragma unroll
     for (int aa = 0; aa < SA; ++aa) {
         __asm__ volatile("" ::: "memory");
         const int a = H * SA + aa;
-        const double *sg = cst + a * 12, *ep = cst + a * 12 + 6;
-        const double u[3] = {sg[0] * f[0] + sg[3] * f[1] + sg[4] * f[2], sg[3] * f[0] + sg[1] * f[1] + sg[5] * f[2],
-                             sg[4] * f[0] + sg[5] * f[1] + sg[2] * f[2]};
-        const double uf = u[0] * f[0] + u[1] * f[1] + u[2] * f[2];
-        const double uwr = u[0] * wr[0] + u[1] * wr[1] + u[2] * wr[2];
-        const double uwi = u[0] * wi[0] + u[1] * wi[1] + u[2] * wi[2];
-        double ser = 0.0, sei = 0.0;
+        const double *ep = cst + a * 12 + 6;
+        double gr = 0.0, gi = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            ser += Sr[c] * ep[c];
-            sei += Si[c] * ep[c];
+            gr += Pr[c] * ep[c];
+            gi += Pi[c] * ep[c];
         }
-        double gr = (ser - (c2 * uwr - c4 * uf * Vr)) * G.invmu;
-        double gi = (sei - (c2 * uwi - c4 * uf * Vi)) * G.invmu;
+        gr *= G.invmu;
+        gi *= G.invmu;
         if (!(f2 > 0)) gr = gi = 0.0;
         sink(aa, gr, gi);
     }
@@ -372,7 +388,7 @@ __global__ void __launch_bounds__(NTHR)
     constexpr int TPP = otf_tpp<R, N, S>();
     constexpr int SA = S / TPP;
     constexpr int NPT = NP * N; // k-points per block
-    __shared__ V lds[N * LDP];
+    PFDD_TILE(V, lds, N * LDP);
     __shared__ double cst[S * 12];
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
@@ -395,9 +411,9 @@ __global__ void __launch_bounds__(NTHR)
             otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
             otf_point<S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, G, [&](int a, double gr, double gi) {
                 const int idx = p * LDP + a * NP + q;
-                const V x = lds[idx];
+                const V x = lds.ld(idx);
                 el += (double)x.x * gr + (double)x.y * gi;
-                lds[idx] = mk<V>((R)gr, (R)gi);
+                lds.st(idx, mk<V>((R)gr, (R)gi));
             });
         }
     } else {
@@ -422,9 +438,9 @@ __global__ void __launch_bounds__(NTHR)
 #pragma unroll
             for (int aa = 0; aa < SA; ++aa) {
                 const int idx = p * LDP + (h * SA + aa) * NP + q;
-                const V x = lds[idx];
+                const V x = lds.ld(idx);
                 el += (double)x.x * g[aa][0] + (double)x.y * g[aa][1];
-                lds[idx] = mk<V>((R)g[aa][0], (R)g[aa][1]);
+                lds.st(idx, mk<V>((R)g[aa][0], (R)g[aa][1]));
             }
         }
     }
@@ -434,7 +450,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds[p * LDP + l];
+        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
     }
     double v[1] = {el};
     block_sum<1, NTHR>(v, red, tid);
@@ -459,7 +475,7 @@ __global__ void __launch_bounds__(NTHR)
     typedef typename cvec<R>::type V;
     if (*done) return;
     constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
-    __shared__ V lds[N * LDP];
+    PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     __shared__ int lsys[NL];
     __shared__ double red[4 * (NTHR / 64)];
@@ -502,7 +518,7 @@ __global__ void __launch_bounds__(NTHR)
         if (b < 0) continue;
         const long long idx = b + p;
         const int a = lsys[l];
-        const V g = lds[p * LDP + l];
+        const V g = lds.ld(p * LDP + l);
         V x;
         if constexpr (PREF) x = xold[t];
         else x = xo[idx];
@@ -1195,6 +1211,8 @@ int build_B(pfdd_gl *h)
             G.eps6[a * 6 + c] = c < 3 ? bp.eps[a * 9 + q] : bp.eps[a * 9 + 3 * r + cc] + bp.eps[a * 9 + 3 * cc + r];
         }
     for (int q = 0; q < 9; ++q) G.four[q] = h->four[q];
+    G.C44x2 = 2.0 * h->C44;
+    G.C12 = h->C12;
     G.inv2C44 = bp.inv2C44;
     G.inv1mnu = bp.inv1mnu;
     G.invmu = bp.invmu;
