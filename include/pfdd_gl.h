@@ -118,8 +118,10 @@ int pfdd_gl_get_xi(pfdd_gl *h, double *xi_host);
  * as above, element type double2 for FP64 or float2 for FP32).  The pointer
  * changes after every GL iteration (ping-pong buffers); fetch it again. */
 int pfdd_gl_xi_device(pfdd_gl *h, void **dptr);
-/* Copy a device buffer (same element type/layout) into the current field. */
+/* Copy a device buffer (same element type/layout) into the current field,
+ * and the current field into a device buffer. */
 int pfdd_gl_set_xi_device(pfdd_gl *h, const void *dptr);
+int pfdd_gl_get_xi_device(pfdd_gl *h, void *dptr);
 
 /* Resolved shear stresses tau_a in units of mu: the result of
  * App::resolSS -> FFTW_Slab::resolSS_Schmid (fft_fftw_slab.cpp:737-769). */
@@ -158,6 +160,28 @@ int pfdd_gl_diag(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
  * current field.  Either output pointer may be NULL. */
 int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
                         double ave_sigma[9], double *data_eps, double *data_sigma);
+
+/* Reverse mode (no counterpart in the reference: it has no derivatives).
+ * pfdd_gl_tape_record runs `nsteps` GL iterations from the current field at
+ * the current tau and core coefficients, exactly as pfdd_gl_step(h, nsteps,
+ * 0, out) does, and keeps a checkpoint of xi every `every` iterations
+ * (ceil(nsteps/every) fields on the device).  pfdd_gl_tape_backward then takes
+ * dL/dxi_T for the final field (layout of fft->xi[0]: Re and Im parts are
+ * independent real variables, i.e. the torch convention for a real loss)
+ * and returns, by recomputing each segment from its checkpoint (`every`-1
+ * more fields) and running the adjoint iteration backwards:
+ *   dxi_0[2*S*N]        dL/dxi_0 (may be NULL)
+ *   dtau[S]             dL/dtau_a (may be NULL)
+ *   dcoef[S*ncoef]      dL/dc_{a,m} of the core-energy coefficients in their
+ *                       sine-series form dE = A sum_m c_m m pi sin(2 m pi x)
+ *                       (PERFECT: m = 1 -> An'; SINE: m = 1, 2 -> isf', Cn';
+ *                       per system -- sum over a for a shared coefficient)
+ * Requires nx*ny to be a multiple of the inverse-z tile (every power-of-two
+ * and 48/96 grid).  The _device variant takes device pointers (double2
+ * layout, any precision handle). */
+int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out);
+int pfdd_gl_tape_backward(pfdd_gl *h, const double *dxi_T, double *dxi_0, double *dtau, double *dcoef);
+int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, double *dtau, double *dcoef);
 
 /* Average device time (ms) of each kernel of one GL iteration over `nsteps`
  * iterations, measured with HIP events on the handle's stream.  ms[0..4] =

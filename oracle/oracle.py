@@ -269,3 +269,44 @@ class Ranmar:
         out = np.empty(int(n))
         lib().orc_ranmar_fill(_p(self.st), int(n), _p(out))
         return out
+
+
+def adjoint_rollout(xi0, BB, cddt, tau, coef, nsteps, dxi_T, amp=None, xo=None):
+    """Reverse-mode derivative of `nsteps` GL iterations with the sine-series
+    core energy (core flag 8: dE = A sum_m c_m m pi sin(2 m pi Re xi)).
+
+    Not a reference function (the reference has no derivatives): the adjoint
+    of orc_gl_step / orc_update (update_order_parameter,
+    fft_fftw_slab.cpp:1746-1790) written out directly in numpy with all
+    forward states held in memory, for small grids.  Re and Im parts are
+    independent real variables.  Returns (dL/dxi_0, dL/dtau[S], dL/dcoef[S, nm]).
+    """
+    coef = np.atleast_2d(_f64(coef))
+    S = xi0.shape[0]
+    nm = coef.shape[1]
+    st = GLState(xi0, BB, cddt, 8, coef.reshape(-1), nm=nm, amp=amp, xo=xo)
+    states = [st.xi.copy()]
+    for _ in range(nsteps):
+        st.step(tau)
+        states.append(st.xi.copy())
+    U = np.ones(xi0.shape) if xo is None else (np.asarray(xo).reshape(xi0.shape) == 0).astype(float)
+    A = np.ones(xi0.shape) if amp is None else np.asarray(amp, dtype=float).reshape(xi0.shape)
+    BBt = np.ascontiguousarray(BB)  # BB[b, a]: ghat_a = sum_b BB[b, a] xihat_b
+    lam = np.array(dxi_T, dtype=np.complex128)
+    gtau = np.zeros(S)
+    gc = np.zeros((S, nm))
+    m = np.arange(1, nm + 1).reshape(nm, 1, 1, 1)
+    for n in range(nsteps - 1, -1, -1):
+        xr = states[n].real
+        mu = U * lam
+        muh = np.fft.fftn(mu, axes=(1, 2, 3))
+        gh = np.einsum("ba...,b...->a...", BBt, muh)
+        g = np.fft.ifftn(gh, axes=(1, 2, 3))  # = IFFT_unnormalised(gh) / N
+        d2 = np.zeros(xi0.shape)
+        for a in range(S):
+            ang = 2.0 * np.pi * m * xr[a][None]
+            d2[a] = A[a] * np.sum(coef[a].reshape(nm, 1, 1, 1) * 2.0 * (m * np.pi) ** 2 * np.cos(ang), axis=0)
+            gc[a] -= cddt * np.sum(mu[a].real[None] * A[a][None] * m * np.pi * np.sin(ang), axis=(1, 2, 3))
+            gtau[a] += cddt * np.sum(mu[a].real)
+        lam = lam - cddt * (g + d2 * mu.real)
+    return lam, gtau, gc, states[-1]

@@ -152,11 +152,11 @@ template <typename R, int N, bool INV>
 __global__ void __launch_bounds__(NTHR)
     k_fft_pass(const typename cvec<R>::type *__restrict__ in, typename cvec<R>::type *__restrict__ out,
                LineMap mi, LineMap mo, long long nlines, const typename cvec<R>::type *__restrict__ tw,
-               const int *__restrict__ done)
+               const int *__restrict__ done, const uint8_t *__restrict__ zmask)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1;
+    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     const int tid = threadIdx.x;
@@ -170,6 +170,15 @@ __global__ void __launch_bounds__(NTHR)
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, in, bin, mi.SP, tid);
     __syncthreads();
+    if (zmask) {
+        // input elements whose zmask byte is set enter the transform as 0
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e / N, p = e % N;
+            const long long b = bin[l];
+            if (b >= 0 && zmask[b + p * mi.SP]) lds.st(p * LDP + l, mk<V>((R)0, (R)0));
+        }
+        __syncthreads();
+    }
     fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
     tile_store<N, NL, NTHR>(out, lds, bout, mo.SP, tid);
 }
@@ -571,6 +580,129 @@ __global__ void __launch_bounds__(NTHR)
     }
 }
 
+// ------------------------------------------ adjoint of pass 5 (reverse mode)
+// One GL iteration (k_update) maps xi to
+//   xi' = xi - cddt U (g/N - tau + dE(Re xi))      (Re part; Im: xi' = xi - cddt U g/N)
+// with U = 1 on unmasked cells and g = IFFT(B FFT xi).  The elastic operator
+// is self-adjoint (B(k) real, symmetric in (a, b) and even in k), so for
+// lam' = dL/dxi' (Re and Im taken as independent real variables)
+//   mu  = U lam'
+//   lam = lam' - cddt (IFFT(B FFT mu)/N + dE'(Re xi) Re mu)   (dE' acts on Re only)
+//   dL/dtau_a   += cddt sum_r Re mu_a
+//   dL/dc_{a,m} += -cddt sum_r Re mu_a A m pi sin(2 m pi Re xi_a)
+// with dE = A sum_m c_m m pi sin(2 m pi x), dE' = A sum_m c_m 2 m^2 pi^2 cos(2 m pi x).
+// T1 holds mu after passes 1-4; this kernel applies the last inverse pass and
+// the update above, and leaves per-block sums of Re mu (slot 0) and of
+// Re mu A m pi sin(2 m pi x) (slot m) in gpart.  Every block covers lines of
+// one system (the host checks nx*ny % NL == 0).
+template <typename R, typename RA, int N>
+__global__ void __launch_bounds__(NTHR)
+    k_adj_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ lam,
+                 const typename cvec<R>::type *__restrict__ xs, typename cvec<R>::type *__restrict__ lam_out,
+                 const uint8_t *__restrict__ mask, const RA *__restrict__ amp, LineMap mi, LineMap mo,
+                 long long nlines, long long lines_per_sys, StepParams P,
+                 const typename cvec<R>::type *__restrict__ tw, double *__restrict__ gpart, long long gstride,
+                 const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int PER = (TOT + NTHR - 1) / NTHR;
+    constexpr int NG = 1 + MAXM;
+    PFDD_TILE(V, lds, N * LDP);
+    __shared__ long long bin[NL], bout[NL];
+    __shared__ double red[NG * (NTHR / 64)];
+    const int tid = threadIdx.x;
+    const long long L0 = (long long)blockIdx.x * NL;
+    for (int l = tid; l < NL; l += NTHR) {
+        const long long L = L0 + l;
+        const bool ok = L < nlines;
+        bin[l] = ok ? line_base(mi, L) : -1;
+        bout[l] = ok ? line_base(mo, L) : -1;
+    }
+    __syncthreads();
+    tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
+    __syncthreads();
+    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+
+    const int a = (int)(L0 / lines_per_sys);
+    const R cddt = (R)P.cddt, invN = (R)P.invN;
+    const int nm = P.ncoef;
+    double acc[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) acc[q] = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < PER; ++t) {
+        const int e = tid + t * NTHR;
+        if (TOT % NTHR != 0 && e >= TOT) continue;
+        const int l = e / N, p = e % N;
+        const long long b = bout[l];
+        if (b < 0) continue;
+        const long long idx = b + p;
+        const V g = lds.ld(p * LDP + l);
+        const V lm = lam[idx];
+        const V x = xs[idx];
+        const R mr = (mask && mask[idx]) ? (R)0 : lm.x;
+        const R A = amp ? (R)amp[idx] : (R)1;
+        // sin, cos of 2 pi m x by the angle-addition recurrence from m = 1
+        R sp, cp;
+        if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
+        else sincospif((float)x.x, (float *)&sp, (float *)&cp);
+        const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
+        R sm = s1, cm = c1, d2 = (R)0;
+#pragma unroll
+        for (int m = 1; m <= MAXM; ++m) {
+            if (m <= nm) {
+                const R c = (R)P.coef[a * MAXM + m - 1];
+                d2 += c * ((R)(2 * m * m) * (R)(M_PI * M_PI)) * cm;
+                acc[m] += (double)mr * (double)A * (double)m * M_PI * (double)sm;
+                const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
+                sm = sn;
+                cm = cn;
+            }
+        }
+        V y;
+        y.x = lm.x - cddt * (g.x * invN + A * d2 * mr);
+        y.y = lm.y - cddt * (g.y * invN);
+        lam_out[idx] = y;
+        acc[0] += (double)mr;
+    }
+    block_sum<NG, NTHR>(acc, red, tid);
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) gpart[q * gstride + blockIdx.x] = acc[q];
+    }
+}
+
+// Adds the per-block adjoint sums of one iteration into gacc[a][slot] in a
+// fixed order: one workgroup per system, blocks [a*nbps, (a+1)*nbps).
+__global__ void __launch_bounds__(NTHR)
+    k_grad_reduce(const double *__restrict__ gpart, long long gstride, int nbps, double *__restrict__ gacc,
+                  const int *__restrict__ done)
+{
+    if (*done) return;
+    constexpr int NG = 1 + MAXM;
+    __shared__ double red[NG * (NTHR / 64)];
+    const int tid = threadIdx.x, a = blockIdx.x;
+    double v[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        double sum = 0.0, c = 0.0;
+        for (int b = tid; b < nbps; b += NTHR) {
+            const double y = gpart[q * gstride + (long long)a * nbps + b] - c;
+            const double t = sum + y;
+            c = (t - sum) - y;
+            sum = t;
+        }
+        v[q] = sum;
+    }
+    block_sum<NG, NTHR>(v, red, tid);
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) gacc[a * NG + q] += v[q];
+    }
+}
+
 // Deterministic reduction of the block partials in two levels: every block
 // sums a fixed chunk (Kahan within a thread, fixed-shape tree across the
 // block) into stage2; the last block to finish (atomic ticket) sums stage2 in
@@ -798,7 +930,7 @@ int tile_lines_rt(int n)
 
 template <typename R, bool INV>
 hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo, long long nlines,
-                       const void *tw, const int *done, hipStream_t st)
+                       const void *tw, const int *done, hipStream_t st, const uint8_t *zmask = nullptr)
 {
     typedef typename cvec<R>::type V;
 #define PFDD_CASE(n_)                                                                           \
@@ -806,7 +938,7 @@ hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo,
         constexpr int NL = tile_lines<R, n_>();                                                 \
         const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
         hipLaunchKernelGGL((k_fft_pass<R, n_, INV>), dim3(nb), dim3(NTHR), 0, st, (const V *)in, \
-                           (V *)out, mi, mo, nlines, (const V *)tw, done);                      \
+                           (V *)out, mi, mo, nlines, (const V *)tw, done, zmask);               \
         return hipGetLastError();                                                               \
     }
     PFDD_SIZES(PFDD_CASE)
@@ -829,6 +961,27 @@ hipError_t launch_update(int n, const void *T1, const void *xo, void *xn, const 
                            (const V *)xo, (V *)xn, mask, prev, (const RA *)amp, mi, mo, nlines, lps, \
                            P, (const V *)tw, partial, pstride, done);                            \
         return hipGetLastError();                                                                \
+    }
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R, typename RA>
+hipError_t launch_adj_update(int n, const void *T1, const void *lam, const void *xs, void *lam_out,
+                             const uint8_t *mask, const void *amp, LineMap mi, LineMap mo, long long nlines,
+                             long long lps, const StepParams &P, const void *tw, double *gpart,
+                             long long gstride, const int *done, hipStream_t st)
+{
+    typedef typename cvec<R>::type V;
+#define PFDD_CASE(n_)                                                                               \
+    if (n == n_) {                                                                                  \
+        constexpr int NL = tile_lines<R, n_>();                                                     \
+        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                     \
+        hipLaunchKernelGGL((k_adj_update<R, RA, n_>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,   \
+                           (const V *)lam, (const V *)xs, (V *)lam_out, mask, (const RA *)amp, mi, mo, \
+                           nlines, lps, P, (const V *)tw, gpart, gstride, done);                    \
+        return hipGetLastError();                                                                   \
     }
     PFDD_SIZES(PFDD_CASE)
 #undef PFDD_CASE
@@ -995,6 +1148,17 @@ struct pfdd_gl {
     void *dw[2] = {nullptr, nullptr}; // output-step field workspace, max(S,6)*N complex each
     double2 *deps = nullptr;          // 9*N strain field (complex)
     double *dsig = nullptr;           // 9*N stress field (real)
+    // reverse mode: checkpoints of a recorded run and the adjoint workspace
+    struct Tape {
+        int nsteps = 0, every = 0;
+        bool valid = false;
+        StepParams P{};             // tau and core coefficients the run used
+        std::vector<void *> ckpt;   // xi at steps 0, every, 2*every, ...
+        std::vector<void *> seg;    // recomputed states of one segment (every - 1)
+        void *lam[2] = {nullptr, nullptr};
+        double *gpart = nullptr;    // (1 + MAXM) * nb5 per-block sums
+        double *gacc = nullptr;     // S * (1 + MAXM) accumulated sums
+    } tape;
 };
 
 namespace {
@@ -1068,12 +1232,12 @@ Maps make_maps(const pfdd_gl *h, long long S)
     return m;
 }
 
+// One GL iteration from xo into T2 (T2 doubles as the temporary of passes
+// 2-4), with the stress and core parameters P.
 template <typename R>
-int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
+int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, const StepParams &P)
 {
     const Maps m = make_maps(h, h->S);
-    void *xo = h->xi[h->cur];
-    void *T2 = h->xi[1 - h->cur];
     hipStream_t st = h->stream;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st)));
@@ -1092,11 +1256,11 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
     hipError_t e;
     if (h->prec == PFDD_FP64)
         e = launch_update<R, double>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
-                                     (long long)h->nx * h->ny, h->P, h->tw[2], h->partial, h->pstride,
+                                     (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride,
                                      h->done, st);
     else
         e = launch_update<R, float>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
-                                    (long long)h->nx * h->ny, h->P, h->tw[2], h->partial, h->pstride,
+                                    (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride,
                                     h->done, st);
     HIPCHK(e);
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
@@ -1105,8 +1269,49 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
                        h->steps);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[6], st));
+    return PFDD_OK;
+}
+
+template <typename R>
+int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
+{
+    const int rc = step_core<R>(h, tol, ev, h->xi[h->cur], h->xi[1 - h->cur], h->P);
+    if (rc) return rc;
     h->cur = 1 - h->cur;
     h->stepped = true;
+    return PFDD_OK;
+}
+
+// One reverse-mode iteration: lam_in = dL/dxi_{n+1} -> lam_out = dL/dxi_n,
+// with xs = xi_n, accumulating dL/dtau and dL/dcoef sums into tape.gacc.
+template <typename R>
+int adjoint_core(pfdd_gl *h, const void *xs, const void *lam_in, void *lam_out, const StepParams &P)
+{
+    const Maps m = make_maps(h, h->S);
+    hipStream_t st = h->stream;
+    HIPCHK((launch_pass<R, false>(h->nz, lam_in, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, h->mask)));
+    HIPCHK((launch_pass<R, false>(h->ny, h->T1, lam_out, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
+    if (h->stored_B)
+        HIPCHK((launch_mid<R>(h->nx, h->S, lam_out, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
+                              h->partial + 4 * h->pstride, h->done, st)));
+    else
+        HIPCHK((launch_mid_otf<R>(h->nx, h->S, lam_out, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
+                                  h->partial + 4 * h->pstride, h->done, st)));
+    HIPCHK((launch_pass<R, true>(h->ny, lam_out, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
+    const long long gstride = h->nb5;
+    hipError_t e;
+    if (h->prec == PFDD_FP64)
+        e = launch_adj_update<R, double>(h->nz, h->T1, lam_in, xs, lam_out, h->mask, h->amp, m.p5i, m.p5o, m.n5,
+                                         (long long)h->nx * h->ny, P, h->tw[2], h->tape.gpart, gstride,
+                                         h->done, st);
+    else
+        e = launch_adj_update<R, float>(h->nz, h->T1, lam_in, xs, lam_out, h->mask, h->amp, m.p5i, m.p5o, m.n5,
+                                        (long long)h->nx * h->ny, P, h->tw[2], h->tape.gpart, gstride,
+                                        h->done, st);
+    HIPCHK(e);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(h->S), dim3(NTHR), 0, st, h->tape.gpart, gstride, h->nb5 / h->S,
+                       h->tape.gacc, h->done);
+    HIPCHK(hipGetLastError());
     return PFDD_OK;
 }
 
@@ -1129,8 +1334,12 @@ void destroy_all(pfdd_gl *h)
     if (h->stream) hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
                     h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket, h->dw[0], h->dw[1],
-                    h->deps, h->dsig};
+                    h->deps, h->dsig, h->tape.lam[0], h->tape.lam[1], h->tape.gpart, h->tape.gacc};
     for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (void *p : h->tape.ckpt)
+        if (p) hipFree(p);
+    for (void *p : h->tape.seg)
         if (p) hipFree(p);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -1346,6 +1555,117 @@ int diag_fields(pfdd_gl *h, const double *sigma_rot, const double *avepst, const
     return PFDD_OK;
 }
 
+int collect_steps(pfdd_gl *h, int cur0, pfdd_gl_reduce *out)
+{
+    double res[NRES];
+    int done = 0, steps = 0;
+    HIPCHK(hipMemcpyAsync(res, h->res, sizeof(res), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&steps, h->steps, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the field moved steps times from buffer cur0
+    h->cur = (steps % 2) ? 1 - cur0 : cur0;
+    if (done == 2)
+        return fail(PFDD_ERR_NONFINITE, "non-finite reduction after %d GL iterations (field diverged)", steps);
+    if (out) {
+        out->xinorm = res[0];
+        out->xiave = res[1];
+        out->e_elastic = res[2];
+        out->e_core = res[3];
+        out->e_stress = res[4];
+        out->e_total = res[5];
+        out->e_core_ref = res[6];
+        out->steps_done = steps;
+        out->converged = done == 1;
+    }
+    return PFDD_OK;
+}
+
+int tape_backward_any(pfdd_gl *h, const void *dxi_T, bool dev_in, void *dxi_0, bool dev_out, double *dtau,
+                             double *dcoef)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    pfdd_gl::Tape &T = h->tape;
+    if (!T.valid) return fail(PFDD_ERR_STATE, "no recorded run (call pfdd_gl_tape_record first)");
+    if (!dxi_T) return fail(PFDD_ERR_ARG, "null dxi_T");
+    const size_t fb = (size_t)h->S * h->N * h->esz;
+    for (int q = 0; q < 2; ++q)
+        if (!T.lam[q]) HIPCHK(hipMalloc(&T.lam[q], fb));
+    while ((int)T.seg.size() < T.every - 1) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, fb));
+        T.seg.push_back(p);
+    }
+    constexpr int NG = 1 + MAXM;
+    if (!T.gpart) HIPCHK(hipMalloc((void **)&T.gpart, sizeof(double) * NG * h->nb5));
+    if (!T.gacc) HIPCHK(hipMalloc((void **)&T.gacc, sizeof(double) * NG * MAXS));
+    const long long n = (long long)h->S * h->N;
+    // seed
+    if (dev_in) {
+        if (h->prec == PFDD_FP64)
+            HIPCHK(hipMemcpyAsync(T.lam[0], dxi_T, fb, hipMemcpyDeviceToDevice, h->stream));
+        else {
+            hipLaunchKernelGGL((k_from_f64<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                               (const double2 *)dxi_T, (float2 *)T.lam[0], n);
+            HIPCHK(hipGetLastError());
+        }
+    } else if ((rc = upload_field_f64(h, (const double *)dxi_T, T.lam[0]))) {
+        return rc;
+    }
+    HIPCHK(hipMemsetAsync(T.gacc, 0, sizeof(double) * NG * MAXS, h->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    HIPCHK(hipGetLastError());
+    int cur = 0;
+    const int nseg = (T.nsteps + T.every - 1) / T.every;
+    std::vector<const void *> xs(T.every);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * T.every, len = std::min(T.every, T.nsteps - s0);
+        xs[0] = T.ckpt[sg];
+        for (int j = 1; j < len; ++j) {
+            rc = h->prec == PFDD_FP64 ? step_core<double>(h, 0.0, nullptr, xs[j - 1], T.seg[j - 1], T.P)
+                                      : step_core<float>(h, 0.0, nullptr, xs[j - 1], T.seg[j - 1], T.P);
+            if (rc) return rc;
+            xs[j] = T.seg[j - 1];
+        }
+        for (int j = len - 1; j >= 0; --j) {
+            rc = h->prec == PFDD_FP64 ? adjoint_core<double>(h, xs[j], T.lam[cur], T.lam[1 - cur], T.P)
+                                      : adjoint_core<float>(h, xs[j], T.lam[cur], T.lam[1 - cur], T.P);
+            if (rc) return rc;
+            cur = 1 - cur;
+        }
+    }
+    double g[NG * MAXS];
+    int done = 0;
+    HIPCHK(hipMemcpyAsync(g, T.gacc, sizeof(g), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (done) return fail(PFDD_ERR_NONFINITE, "non-finite state while recomputing the recorded run");
+    const double c = T.P.cddt;
+    for (int a = 0; a < h->S; ++a) {
+        if (dtau) dtau[a] = c * g[a * NG];
+        for (int m = 1; m <= T.P.ncoef; ++m)
+            if (dcoef) dcoef[a * T.P.ncoef + m - 1] = -c * g[a * NG + m];
+    }
+    for (int q = 0; q < NG * h->S; ++q)
+        if (!std::isfinite(g[q])) return fail(PFDD_ERR_NONFINITE, "non-finite adjoint sums");
+    if (dxi_0) {
+        if (dev_out) {
+            if (h->prec == PFDD_FP64)
+                HIPCHK(hipMemcpyAsync(dxi_0, T.lam[cur], fb, hipMemcpyDeviceToDevice, h->stream));
+            else {
+                hipLaunchKernelGGL((k_to_f64<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                                   (const float2 *)T.lam[cur], (double2 *)dxi_0, n);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipStreamSynchronize(h->stream));
+        } else if ((rc = download_field_f64(h, T.lam[cur], (double *)dxi_0))) {
+            return rc;
+        }
+    }
+    return PFDD_OK;
+}
+
 } // namespace
 
 // ===================================================================== C ABI
@@ -1536,6 +1856,16 @@ int pfdd_gl_set_xi_device(pfdd_gl *h, const void *src)
     return PFDD_OK;
 }
 
+int pfdd_gl_get_xi_device(pfdd_gl *h, void *dst)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!dst) return fail(PFDD_ERR_ARG, "null destination");
+    HIPCHK(hipMemcpyAsync(dst, h->xi[h->cur], (size_t)h->S * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PFDD_OK;
+}
+
 int pfdd_gl_set_tau(pfdd_gl *h, const double *tau)
 {
     int rc;
@@ -1565,28 +1895,57 @@ int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out)
         if (rc) return rc;
     }
     if (!out && tol <= 0) return PFDD_OK;
-    double res[NRES];
-    int done = 0, steps = 0;
-    HIPCHK(hipMemcpyAsync(res, h->res, sizeof(res), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(&steps, h->steps, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    // the field moved steps times from buffer cur0
-    h->cur = (steps % 2) ? 1 - cur0 : cur0;
-    if (done == 2)
-        return fail(PFDD_ERR_NONFINITE, "non-finite reduction after %d GL iterations (field diverged)", steps);
-    if (out) {
-        out->xinorm = res[0];
-        out->xiave = res[1];
-        out->e_elastic = res[2];
-        out->e_core = res[3];
-        out->e_stress = res[4];
-        out->e_total = res[5];
-        out->e_core_ref = res[6];
-        out->steps_done = steps;
-        out->converged = done == 1;
+    return collect_steps(h, cur0, out);
+}
+
+int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
+    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
+    if (((long long)h->nx * h->ny) % nl != 0)
+        return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);
+    pfdd_gl::Tape &T = h->tape;
+    T.valid = false;
+    const size_t fb = (size_t)h->S * h->N * h->esz;
+    const size_t nck = (size_t)((nsteps + every - 1) / every);
+    while (T.ckpt.size() > nck) {
+        HIPCHK(hipFree(T.ckpt.back()));
+        T.ckpt.pop_back();
     }
+    while (T.ckpt.size() < nck) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, fb));
+        T.ckpt.push_back(p);
+    }
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    HIPCHK(hipGetLastError());
+    const int cur0 = h->cur;
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % every == 0)
+            HIPCHK(hipMemcpyAsync(T.ckpt[s / every], h->xi[h->cur], fb, hipMemcpyDeviceToDevice, h->stream));
+        rc = h->prec == PFDD_FP64 ? step_once<double>(h, 0.0, nullptr) : step_once<float>(h, 0.0, nullptr);
+        if (rc) return rc;
+    }
+    pfdd_gl_reduce r{};
+    if ((rc = collect_steps(h, cur0, &r))) return rc;
+    if (out) *out = r;
+    T.nsteps = nsteps;
+    T.every = every;
+    T.P = h->P;
+    T.valid = true;
     return PFDD_OK;
+}
+
+int pfdd_gl_tape_backward(pfdd_gl *h, const double *dxi_T, double *dxi_0, double *dtau, double *dcoef)
+{
+    return tape_backward_any(h, dxi_T, false, dxi_0, false, dtau, dcoef);
+}
+
+int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, double *dtau, double *dcoef)
+{
+    return tape_backward_any(h, dxi_T, true, dxi_0, true, dtau, dcoef);
 }
 
 int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)

@@ -66,8 +66,10 @@ class GLReduce(ctypes.Structure):
 EXPORTS = [
     "pfdd_gl_version", "pfdd_gl_last_error", "pfdd_gl_create", "pfdd_gl_destroy",
     "pfdd_gl_set_xi", "pfdd_gl_get_xi", "pfdd_gl_xi_device", "pfdd_gl_set_xi_device",
+    "pfdd_gl_get_xi_device",
     "pfdd_gl_set_tau", "pfdd_gl_set_core_coef", "pfdd_gl_step", "pfdd_gl_xi_sums",
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
+    "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
 ]
 
 _lib = None
@@ -93,6 +95,7 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_get_xi": (c_int, [vp, _dp]),
         "pfdd_gl_xi_device": (c_int, [vp, ctypes.POINTER(vp)]),
         "pfdd_gl_set_xi_device": (c_int, [vp, vp]),
+        "pfdd_gl_get_xi_device": (c_int, [vp, vp]),
         "pfdd_gl_set_tau": (c_int, [vp, _dp]),
         "pfdd_gl_set_core_coef": (c_int, [vp, c_int, _dp]),
         "pfdd_gl_step": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
@@ -101,6 +104,9 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_diag_fields": (c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "pfdd_gl_profile": (c_int, [vp, c_int, _dp, ip]),
         "pfdd_gl_traffic": (c_int, [vp, _dp, _dp]),
+        "pfdd_gl_tape_record": (c_int, [vp, c_int, c_int, ctypes.POINTER(GLReduce)]),
+        "pfdd_gl_tape_backward": (c_int, [vp, _dp, _dp, _dp, _dp]),
+        "pfdd_gl_tape_backward_device": (c_int, [vp, vp, vp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -192,6 +198,9 @@ class GLSolver:
     def set_xi_device(self, ptr: int):
         _check(self._lib.pfdd_gl_set_xi_device(self._h, ctypes.c_void_p(ptr)))
 
+    def get_xi_device(self, ptr: int):
+        _check(self._lib.pfdd_gl_get_xi_device(self._h, ctypes.c_void_p(ptr)))
+
     def set_tau(self, tau):
         t = _f64(tau)
         assert t.size == self.S
@@ -207,6 +216,32 @@ class GLSolver:
         _check(self._lib.pfdd_gl_step(self._h, int(nsteps), float(tol),
                                       ctypes.byref(r) if sync else None))
         return r.as_dict() if sync else None
+
+    # --------------------------------------------------------- reverse mode
+    def tape_record(self, nsteps, every):
+        """nsteps GL iterations (as step(nsteps)) keeping checkpoints every `every`."""
+        r = GLReduce()
+        _check(self._lib.pfdd_gl_tape_record(self._h, int(nsteps), int(every), ctypes.byref(r)))
+        return r.as_dict()
+
+    def tape_backward(self, dxi_T, ncoef):
+        """dL/dxi_T (complex, Re/Im independent) -> (dL/dxi_0, dL/dtau[S], dL/dcoef[S, ncoef])."""
+        g = np.ascontiguousarray(dxi_T, dtype=np.complex128)
+        assert g.shape == (self.S,) + self.shape, g.shape
+        dxi0 = np.empty_like(g)
+        dtau = np.zeros(self.S)
+        dcoef = np.zeros(self.S * max(int(ncoef), 1))
+        _check(self._lib.pfdd_gl_tape_backward(self._h, g.ctypes.data_as(_dp), dxi0.ctypes.data_as(_dp),
+                                               _ptr(dtau), _ptr(dcoef)))
+        return dxi0, dtau, dcoef[: self.S * int(ncoef)].reshape(self.S, int(ncoef))
+
+    def tape_backward_device(self, dxi_T_ptr: int, dxi_0_ptr: int, ncoef):
+        """As tape_backward with device buffers (complex128 layout)."""
+        dtau = np.zeros(self.S)
+        dcoef = np.zeros(self.S * max(int(ncoef), 1))
+        _check(self._lib.pfdd_gl_tape_backward_device(self._h, ctypes.c_void_p(dxi_T_ptr),
+                                                      ctypes.c_void_p(dxi_0_ptr), _ptr(dtau), _ptr(dcoef)))
+        return dtau, dcoef[: self.S * int(ncoef)].reshape(self.S, int(ncoef))
 
     def xi_sums(self):
         out = np.zeros(self.S)
