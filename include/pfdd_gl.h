@@ -105,6 +105,17 @@ const char *pfdd_gl_last_error(void);
  * device in FP64.  The initial xi is zero. */
 int pfdd_gl_create(const pfdd_gl_desc *desc, pfdd_gl **out);
 
+/* A batch of `nbatch` independent runs of the same grid, slip systems and
+ * material in one handle (no counterpart in the reference, which runs one
+ * grid per process): the fields of all members are concatenated, member m's
+ * systems being m*ns .. m*ns+ns-1, and every kernel of a GL iteration covers
+ * all members in one launch.  Per member: initial field, tau (set_tau takes
+ * nbatch*ns values), amp and xo (nbatch*ns*N values); shared: core
+ * coefficients.  Reductions of pfdd_gl_step are sums over all members;
+ * per-member sums of Re xi come from pfdd_gl_xi_sums (nbatch*ns values).
+ * Output-step diagnostics and reverse mode are single-run only. */
+int pfdd_gl_create_batch(const pfdd_gl_desc *desc, int nbatch, pfdd_gl **out);
+
 /* Frees everything the handle owns.  (The reference never frees its FFT
  * arrays: ~FFTW_Slab is empty, fft_fftw_slab.cpp:43-46.) */
 int pfdd_gl_destroy(pfdd_gl *h);
@@ -193,6 +204,19 @@ int pfdd_gl_profile(pfdd_gl *h, int nsteps, double *ms, int *nkernels);
 /* Bytes the GL iteration moves, by the kernel split above (algorithmic
  * figure: each field element read/written once per kernel, B once). */
 int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step);
+
+/* Several handles (e.g. batches of different grid sizes) stepped together by
+ * one hipGraph of `iters` (even) GL iterations, replayed nsteps/iters times
+ * (the remainder runs eagerly); no host synchronisation inside the run.  No
+ * counterpart in the reference.  The graph is captured on first use and again
+ * after set_core_coef, a single-run handle's set_tau, a change of tol or an
+ * odd number of steps taken outside the graph.  pfdd_gl_graph_run reports
+ * each handle's last iteration in out[i] (may be NULL); tol as in
+ * pfdd_gl_step, per handle. */
+typedef struct pfdd_gl_graph pfdd_gl_graph;
+int pfdd_gl_graph_create(pfdd_gl *const *handles, int n, int iters, pfdd_gl_graph **out);
+int pfdd_gl_graph_run(pfdd_gl_graph *g, int nsteps, double tol, pfdd_gl_reduce *out);
+int pfdd_gl_graph_destroy(pfdd_gl_graph *g);
 
 #ifdef __cplusplus
 }

@@ -68,12 +68,13 @@ int fail(int code, const char *fmt, ...)
     } while (0)
 
 struct StepParams {
-    double tau[MAXS];
-    double coef[MAXS * MAXM];
+    double tau[MAXS];          // resolved shear per system (single-run handles)
+    const double *taud;        // device tau per system of a batch (nullptr: use tau[])
+    double coef[MAXS * MAXM];  // core coefficients per system of one member
     double cddt;
     double invN;
     int ncoef;
-    int S;
+    int S;                     // systems per member
 };
 
 struct BParams {
@@ -226,7 +227,7 @@ template <typename R, int N, int S> constexpr int mid_pairs()
 template <typename R, typename RB, int N, int S>
 __global__ void __launch_bounds__(NTHR)
     k_mid(typename cvec<R>::type *__restrict__ T2, const RB *__restrict__ B, long long ncol,
-          long long ntot, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
+          long long ntot, int bpm, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
           const int *__restrict__ done)
 {
     typedef typename cvec<R>::type V;
@@ -237,7 +238,10 @@ __global__ void __launch_bounds__(NTHR)
     PFDD_TILE(V, lds, N * LDP);
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
-    const long long c0 = (long long)blockIdx.x * NP;
+    // member of a batch (bpm blocks per member), first column of the block
+    const int mem = blockIdx.x / bpm;
+    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
+    T2 += (long long)mem * S * ntot;
     // load: lines (a, q) -> T2[a][c0+q][0..N)
 #pragma unroll 4
     for (int e = tid; e < TOT; e += NTHR) {
@@ -386,7 +390,7 @@ __device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p,
 
 template <typename R, int N, int S>
 __global__ void __launch_bounds__(NTHR)
-    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot,
+    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
               const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
               const int *__restrict__ done)
 {
@@ -401,7 +405,10 @@ __global__ void __launch_bounds__(NTHR)
     __shared__ double cst[S * 12];
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
-    const long long c0 = (long long)blockIdx.x * NP;
+    // member of a batch (bpm blocks per member), first column of the block
+    const int mem = blockIdx.x / bpm;
+    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
+    T2 += (long long)mem * S * ntot;
     for (int c = tid; c < S * 12; c += NTHR) cst[c] = (c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
                                                                     : G.eps6[(c / 12) * 6 + c % 12 - 6];
     tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
@@ -486,7 +493,7 @@ __global__ void __launch_bounds__(NTHR)
     constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
-    __shared__ int lsys[NL];
+    __shared__ int lsys[NL], lcs[NL]; // system of the line in the batch / within its member
     __shared__ double red[4 * (NTHR / 64)];
     const int tid = threadIdx.x;
     const long long L0 = (long long)blockIdx.x * NL;
@@ -496,6 +503,7 @@ __global__ void __launch_bounds__(NTHR)
         bin[l] = ok ? line_base(mi, L) : -1;
         bout[l] = ok ? line_base(mo, L) : -1;
         lsys[l] = ok ? (int)(L / lines_per_sys) : 0;
+        lcs[l] = lsys[l] % P.S;
     }
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
@@ -526,7 +534,7 @@ __global__ void __launch_bounds__(NTHR)
         const long long b = bout[l];
         if (b < 0) continue;
         const long long idx = b + p;
-        const int a = lsys[l];
+        const int a = lsys[l], as = lcs[l];
         const V g = lds.ld(p * LDP + l);
         V x;
         if constexpr (PREF) x = xold[t];
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(NTHR)
         else sincospif((float)x.x, (float *)&sp, (float *)&cp);
         R sm = sp, cm = cp, dens = (R)0, dE = (R)0;
         for (int m = 1; m <= nm; ++m) {
-            const R c = (R)P.coef[a * MAXM + m - 1];
+            const R c = (R)P.coef[as * MAXM + m - 1];
             dens += c * sm * sm;
             dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
             const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
@@ -548,7 +556,7 @@ __global__ void __launch_bounds__(NTHR)
         if (amp) A = (R)amp[idx];
         dens *= A;
         dE *= A;
-        const R tau = (R)P.tau[a];
+        const R tau = (R)(P.taud ? P.taud[a] : P.tau[a]);
         V y;
         double dn;
         if (mask && mask[idx]) {
@@ -625,7 +633,7 @@ __global__ void __launch_bounds__(NTHR)
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
 
-    const int a = (int)(L0 / lines_per_sys);
+    const int a = (int)(L0 / lines_per_sys) % P.S;
     const R cddt = (R)P.cddt, invN = (R)P.invN;
     const int nm = P.ncoef;
     double acc[NG];
@@ -1017,7 +1025,7 @@ template <typename R, int N, int S> constexpr bool mid_fits()
 }
 
 template <typename R, int S>
-hipError_t launch_mid_s(int n, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
+hipError_t launch_mid_s(int n, int M, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
                         double *part_el, const int *done, hipStream_t st)
 {
     typedef typename cvec<R>::type V;
@@ -1025,9 +1033,9 @@ hipError_t launch_mid_s(int n, void *T2, const void *B, long long ncol, long lon
     if (n == n_) {                                                                                \
         if constexpr (mid_fits<R, n_, S>()) {                                                     \
             constexpr int NP = mid_pairs<R, n_, S>();                                             \
-            const unsigned nb = (unsigned)((ncol + NP - 1) / NP);                                 \
-            hipLaunchKernelGGL((k_mid<R, R, n_, S>), dim3(nb), dim3(NTHR), 0, st, (V *)T2,        \
-                               (const R *)B, ncol, ntot, (const V *)tw, part_el, done);           \
+            const int bpm = (int)((ncol + NP - 1) / NP);                                          \
+            hipLaunchKernelGGL((k_mid<R, R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), 0, st, \
+                               (V *)T2, (const R *)B, ncol, ntot, bpm, (const V *)tw, part_el, done); \
             return hipGetLastError();                                                             \
         } else {                                                                                  \
             return hipErrorInvalidValue;                                                          \
@@ -1060,32 +1068,32 @@ bool mid_supported(int n, int S)
 }
 
 template <typename R>
-hipError_t launch_mid(int n, int S, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
+hipError_t launch_mid(int n, int S, int M, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
                       double *part_el, const int *done, hipStream_t st)
 {
     switch (S) {
-    case 1: return launch_mid_s<R, 1>(n, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 2: return launch_mid_s<R, 2>(n, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 3: return launch_mid_s<R, 3>(n, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 4: return launch_mid_s<R, 4>(n, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 6: return launch_mid_s<R, 6>(n, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 12: return launch_mid_s<R, 12>(n, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 1: return launch_mid_s<R, 1>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 2: return launch_mid_s<R, 2>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 3: return launch_mid_s<R, 3>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 4: return launch_mid_s<R, 4>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 6: return launch_mid_s<R, 6>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
+    case 12: return launch_mid_s<R, 12>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <typename R, int S>
-hipError_t launch_mid_otf_s(int n, void *T2, const GParams &G, long long ncol, long long ntot, const void *tw,
-                            double *part_el, const int *done, hipStream_t st)
+hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long ncol, long long ntot,
+                            const void *tw, double *part_el, const int *done, hipStream_t st)
 {
     typedef typename cvec<R>::type V;
 #define PFDD_CASE(n_)                                                                             \
     if (n == n_) {                                                                                \
         if constexpr (mid_fits<R, n_, S>()) {                                                     \
             constexpr int NP = mid_pairs<R, n_, S>();                                             \
-            const unsigned nb = (unsigned)((ncol + NP - 1) / NP);                                 \
-            hipLaunchKernelGGL((k_mid_otf<R, n_, S>), dim3(nb), dim3(NTHR), 0, st, (V *)T2, G,    \
-                               ncol, ntot, (const V *)tw, part_el, done);                         \
+            const int bpm = (int)((ncol + NP - 1) / NP);                                          \
+            hipLaunchKernelGGL((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), 0, st, \
+                               (V *)T2, G, ncol, ntot, bpm, (const V *)tw, part_el, done);        \
             return hipGetLastError();                                                             \
         } else {                                                                                  \
             return hipErrorInvalidValue;                                                          \
@@ -1097,16 +1105,16 @@ hipError_t launch_mid_otf_s(int n, void *T2, const GParams &G, long long ncol, l
 }
 
 template <typename R>
-hipError_t launch_mid_otf(int n, int S, void *T2, const GParams &G, long long ncol, long long ntot,
+hipError_t launch_mid_otf(int n, int S, int M, void *T2, const GParams &G, long long ncol, long long ntot,
                           const void *tw, double *part_el, const int *done, hipStream_t st)
 {
     switch (S) {
-    case 1: return launch_mid_otf_s<R, 1>(n, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 2: return launch_mid_otf_s<R, 2>(n, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 3: return launch_mid_otf_s<R, 3>(n, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 4: return launch_mid_otf_s<R, 4>(n, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 6: return launch_mid_otf_s<R, 6>(n, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 12: return launch_mid_otf_s<R, 12>(n, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 1: return launch_mid_otf_s<R, 1>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 2: return launch_mid_otf_s<R, 2>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 3: return launch_mid_otf_s<R, 3>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 4: return launch_mid_otf_s<R, 4>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 6: return launch_mid_otf_s<R, 6>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 12: return launch_mid_otf_s<R, 12>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1116,6 +1124,9 @@ hipError_t launch_mid_otf(int n, int S, void *T2, const GParams &G, long long nc
 // ================================================================== handle
 struct pfdd_gl {
     int nx = 0, ny = 0, nz = 0, S = 0, prec = PFDD_FP64;
+    int M = 1, SM = 0;      // members of a batch, systems over all members (S*M)
+    double *taud = nullptr; // device tau[SM] of a batch
+    unsigned version = 0;   // bumped when kernel arguments change (tau, core coefficients)
     long long N = 0;
     size_t esz = 16; // bytes per complex element
     int device = 0;
@@ -1237,7 +1248,7 @@ Maps make_maps(const pfdd_gl *h, long long S)
 template <typename R>
 int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, const StepParams &P)
 {
-    const Maps m = make_maps(h, h->S);
+    const Maps m = make_maps(h, h->SM);
     hipStream_t st = h->stream;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st)));
@@ -1245,10 +1256,10 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     if (h->stored_B)
-        HIPCHK((launch_mid<R>(h->nx, h->S, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
+        HIPCHK((launch_mid<R>(h->nx, h->S, h->M, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
                               h->partial + 4 * h->pstride, h->done, st)));
     else
-        HIPCHK((launch_mid_otf<R>(h->nx, h->S, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
+        HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
                                   h->partial + 4 * h->pstride, h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
@@ -1287,15 +1298,15 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
 template <typename R>
 int adjoint_core(pfdd_gl *h, const void *xs, const void *lam_in, void *lam_out, const StepParams &P)
 {
-    const Maps m = make_maps(h, h->S);
+    const Maps m = make_maps(h, h->SM);
     hipStream_t st = h->stream;
     HIPCHK((launch_pass<R, false>(h->nz, lam_in, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, h->mask)));
     HIPCHK((launch_pass<R, false>(h->ny, h->T1, lam_out, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
     if (h->stored_B)
-        HIPCHK((launch_mid<R>(h->nx, h->S, lam_out, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
+        HIPCHK((launch_mid<R>(h->nx, h->S, h->M, lam_out, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
                               h->partial + 4 * h->pstride, h->done, st)));
     else
-        HIPCHK((launch_mid_otf<R>(h->nx, h->S, lam_out, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
+        HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, lam_out, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
                                   h->partial + 4 * h->pstride, h->done, st)));
     HIPCHK((launch_pass<R, true>(h->ny, lam_out, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
     const long long gstride = h->nb5;
@@ -1332,7 +1343,7 @@ void destroy_all(pfdd_gl *h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
+    void *ptrs[] = {h->taud, h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
                     h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket, h->dw[0], h->dw[1],
                     h->deps, h->dsig, h->tape.lam[0], h->tape.lam[1], h->tape.gpart, h->tape.gacc};
     for (void *p : ptrs)
@@ -1439,7 +1450,7 @@ int build_B(pfdd_gl *h)
 
 int upload_field_f64(pfdd_gl *h, const double *host, void *dst)
 {
-    const long long n = (long long)h->S * h->N;
+    const long long n = (long long)h->SM * h->N;
     if (h->prec == PFDD_FP64) {
         HIPCHK(hipMemcpyAsync(dst, host, (size_t)n * 16, hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -1463,7 +1474,7 @@ int upload_field_f64(pfdd_gl *h, const double *host, void *dst)
 
 int download_field_f64(pfdd_gl *h, const void *src, double *host)
 {
-    const long long n = (long long)h->S * h->N;
+    const long long n = (long long)h->SM * h->N;
     if (h->prec == PFDD_FP64) {
         HIPCHK(hipMemcpyAsync(host, src, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -1675,11 +1686,15 @@ int pfdd_gl_version(void) { return (0 << 16) | 1; }
 
 const char *pfdd_gl_last_error(void) { return g_err.c_str(); }
 
-int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
+int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out) { return pfdd_gl_create_batch(d, 1, out); }
+
+int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
 {
     g_err.clear();
     if (!d || !out) return fail(PFDD_ERR_ARG, "null descriptor or output");
     *out = nullptr;
+    if (nbatch < 1) return fail(PFDD_ERR_ARG, "nbatch must be >= 1");
+    if ((long long)nbatch * d->ns > 4096) return fail(PFDD_ERR_ARG, "nbatch * ns must be <= 4096");
     if (!size_supported(d->nx) || !size_supported(d->ny) || !size_supported(d->nz))
         return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each dimension must be one of 8,16,32,48,64,96,128,256,512",
                     d->nx, d->ny, d->nz);
@@ -1699,6 +1714,8 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
     h->ny = d->ny;
     h->nz = d->nz;
     h->S = d->ns;
+    h->M = nbatch;
+    h->SM = nbatch * d->ns;
     h->prec = d->precision;
     h->N = (long long)d->nx * d->ny * d->nz;
     h->esz = d->precision == PFDD_FP64 ? 16 : 8;
@@ -1739,7 +1756,7 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
         if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
         h->own_stream = true;
     }
-    const size_t fbytes = (size_t)h->S * h->N * h->esz;
+    const size_t fbytes = (size_t)h->SM * h->N * h->esz;
     auto alloc = [&](void **p, size_t bytes) -> int {
         hipError_t ee = hipMalloc(p, bytes);
         if (ee != hipSuccess)
@@ -1753,11 +1770,17 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
     if ((rc = upload_twiddles(h))) return bail(rc);
 
     // per-block partials
-    const long long n5 = (long long)h->S * h->nx * h->ny;
+    const long long n5 = (long long)h->SM * h->nx * h->ny;
     const int nl5 = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
     h->nb5 = (int)((n5 + nl5 - 1) / nl5);
-    h->nb3 = h->prec == PFDD_FP64 ? mid_blocks<double>(h->nx, h->S, (long long)h->ny * h->nz)
-                                  : mid_blocks<float>(h->nx, h->S, (long long)h->ny * h->nz);
+    h->nb3 = h->M * (h->prec == PFDD_FP64 ? mid_blocks<double>(h->nx, h->S, (long long)h->ny * h->nz)
+                                          : mid_blocks<float>(h->nx, h->S, (long long)h->ny * h->nz));
+    if (h->M > 1) {
+        if ((rc = alloc((void **)&h->taud, sizeof(double) * h->SM))) return bail(rc);
+        e = hipMemsetAsync(h->taud, 0, sizeof(double) * h->SM, h->stream);
+        if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
+        h->P.taud = h->taud;
+    }
     h->pstride = h->nb5 > h->nb3 ? h->nb5 : h->nb3;
     if ((rc = alloc((void **)&h->partial, sizeof(double) * NSLOT * h->pstride))) return bail(rc);
     if ((rc = alloc((void **)&h->res, sizeof(double) * NRES))) return bail(rc);
@@ -1770,14 +1793,15 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
         return bail(rc);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
 
-    // obstacle mask + quirk Q3 predecessor table
+    // obstacle mask + quirk Q3 predecessor table (restarting with every member)
     if (d->xo) {
-        const long long n = (long long)h->S * h->N;
+        const long long n = (long long)h->SM * h->N, nmem = (long long)h->S * h->N;
         std::vector<uint8_t> m(n);
         std::vector<long long> pv(n);
         long long last = -1;
         bool any = false;
         for (long long q = 0; q < n; ++q) {
+            if (q % nmem == 0) last = -1;
             m[q] = d->xo[q] != 0.0;
             any |= m[q] != 0;
             pv[q] = last;
@@ -1792,7 +1816,7 @@ int pfdd_gl_create(const pfdd_gl_desc *d, pfdd_gl **out)
         }
     }
     if (d->amp) {
-        const long long n = (long long)h->S * h->N;
+        const long long n = (long long)h->SM * h->N;
         if (h->prec == PFDD_FP64) {
             if ((rc = alloc(&h->amp, n * sizeof(double)))) return bail(rc);
             if (hipMemcpy(h->amp, d->amp, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
@@ -1850,7 +1874,7 @@ int pfdd_gl_set_xi_device(pfdd_gl *h, const void *src)
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!src) return fail(PFDD_ERR_ARG, "null source");
-    HIPCHK(hipMemcpyAsync(h->xi[h->cur], src, (size_t)h->S * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->xi[h->cur], src, (size_t)h->SM * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->stepped = false;
     return PFDD_OK;
@@ -1861,7 +1885,7 @@ int pfdd_gl_get_xi_device(pfdd_gl *h, void *dst)
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!dst) return fail(PFDD_ERR_ARG, "null destination");
-    HIPCHK(hipMemcpyAsync(dst, h->xi[h->cur], (size_t)h->S * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dst, h->xi[h->cur], (size_t)h->SM * h->N * h->esz, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return PFDD_OK;
 }
@@ -1871,7 +1895,14 @@ int pfdd_gl_set_tau(pfdd_gl *h, const double *tau)
     int rc;
     if ((rc = check_handle(h))) return rc;
     if (!tau) return fail(PFDD_ERR_ARG, "null tau");
+    if (h->M > 1) {
+        if ((rc = set_device(h))) return rc;
+        HIPCHK(hipMemcpyAsync(h->taud, tau, sizeof(double) * h->SM, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PFDD_OK;
+    }
     for (int a = 0; a < h->S; ++a) h->P.tau[a] = tau[a];
+    ++h->version;
     return PFDD_OK;
 }
 
@@ -1879,6 +1910,7 @@ int pfdd_gl_set_core_coef(pfdd_gl *h, int ncoef, const double *coef)
 {
     int rc;
     if ((rc = check_handle(h))) return rc;
+    ++h->version;
     return set_coef(h, PFDD_CORE_SERIES, ncoef, coef, h->amp != nullptr);
 }
 
@@ -1903,12 +1935,13 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
+    if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
     if (((long long)h->nx * h->ny) % nl != 0)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);
     pfdd_gl::Tape &T = h->tape;
     T.valid = false;
-    const size_t fb = (size_t)h->S * h->N * h->esz;
+    const size_t fb = (size_t)h->SM * h->N * h->esz;
     const size_t nck = (size_t)((nsteps + every - 1) / every);
     while (T.ckpt.size() > nck) {
         HIPCHK(hipFree(T.ckpt.back()));
@@ -1955,19 +1988,19 @@ int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)
     if (!sum_re) return fail(PFDD_ERR_ARG, "null output");
     const int nb = 256;
     double *part = nullptr;
-    HIPCHK(hipMalloc(&part, sizeof(double) * nb * h->S));
+    HIPCHK(hipMalloc(&part, sizeof(double) * nb * h->SM));
     if (h->prec == PFDD_FP64)
-        hipLaunchKernelGGL((k_sys_sums<double>), dim3(nb, h->S), dim3(NTHR), 0, h->stream,
+        hipLaunchKernelGGL((k_sys_sums<double>), dim3(nb, h->SM), dim3(NTHR), 0, h->stream,
                            (const double2 *)h->xi[h->cur], h->N, part);
     else
-        hipLaunchKernelGGL((k_sys_sums<float>), dim3(nb, h->S), dim3(NTHR), 0, h->stream,
+        hipLaunchKernelGGL((k_sys_sums<float>), dim3(nb, h->SM), dim3(NTHR), 0, h->stream,
                            (const float2 *)h->xi[h->cur], h->N, part);
     HIPCHK(hipGetLastError());
-    std::vector<double> hp((size_t)nb * h->S);
-    HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb * h->S, hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> hp((size_t)nb * h->SM);
+    HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * nb * h->SM, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(part));
-    for (int a = 0; a < h->S; ++a) {
+    for (int a = 0; a < h->SM; ++a) {
         double s = 0.0, c = 0.0;
         for (int b = 0; b < nb; ++b) {
             const double y = hp[(size_t)a * nb + b] - c;
@@ -1987,6 +2020,7 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], 
     int rc;
     if ((rc = check_handle(h))) return rc;
     if (!sigma_rot || !ave_eps || !ave_sigma) return fail(PFDD_ERR_ARG, "null argument");
+    if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "output-step diagnostics of a batch handle");
     std::vector<double> sums(h->S);
     if ((rc = pfdd_gl_xi_sums(h, sums.data()))) return rc;
     const double nsize = (double)h->N;
@@ -2071,7 +2105,7 @@ int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step
 {
     int rc;
     if ((rc = check_handle(h))) return rc;
-    const double f = (double)h->S * h->N * h->esz;
+    const double f = (double)h->SM * h->N * h->esz;
     const double bsz =
         h->stored_B ? (double)h->S * (h->S + 1) / 2 * h->N * (h->prec == PFDD_FP64 ? 8 : 4) : 0.0;
     const double k[5] = {2 * f, 2 * f, 2 * f + bsz, 2 * f, 3 * f};
@@ -2081,6 +2115,154 @@ int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step
         tot += k[q];
     }
     if (bytes_per_step) *bytes_per_step = tot;
+    return PFDD_OK;
+}
+
+} // extern "C"
+
+// ================================================================== graphs
+// A hipGraph holding `iters` (even) GL iterations of several handles, replayed
+// as one launch.  Kernel arguments are frozen at capture: the graph is
+// captured again when a handle's kernel arguments changed (set_tau of a
+// single-run handle, set_core_coef), when its current ping-pong buffer is not
+// the one captured, or when tol changes.  A batch handle reads tau from
+// device memory, so set_tau needs no new capture.
+struct pfdd_gl_graph {
+    std::vector<pfdd_gl *> hs;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned> ver;
+    std::vector<int> cur;
+    int iters = 2;
+    double tol = 0.0;
+};
+
+namespace {
+
+void graph_release(pfdd_gl_graph *g)
+{
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
+int graph_capture(pfdd_gl_graph *g, double tol)
+{
+    graph_release(g);
+    std::vector<hipStream_t> saved(g->hs.size());
+    for (size_t i = 0; i < g->hs.size(); ++i) {
+        saved[i] = g->hs[i]->stream;
+        g->hs[i]->stream = g->stream;
+    }
+    int rc = PFDD_OK;
+    hipError_t e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        rc = fail(PFDD_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    } else {
+        for (int it = 0; it < g->iters && !rc; ++it)
+            for (pfdd_gl *h : g->hs) {
+                rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
+                if (rc) break;
+            }
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(g->stream, &graph);
+        if (!rc && e != hipSuccess) rc = fail(PFDD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        g->graph = graph;
+    }
+    for (size_t i = 0; i < g->hs.size(); ++i) g->hs[i]->stream = saved[i];
+    if (rc) {
+        graph_release(g);
+        return rc;
+    }
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->tol = tol;
+    for (size_t i = 0; i < g->hs.size(); ++i) {
+        g->ver[i] = g->hs[i]->version;
+        g->cur[i] = g->hs[i]->cur;
+    }
+    return PFDD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pfdd_gl_graph_create(pfdd_gl *const *handles, int n, int iters, pfdd_gl_graph **out)
+{
+    g_err.clear();
+    if (!handles || n < 1 || !out) return fail(PFDD_ERR_ARG, "need n >= 1 handles and an output");
+    if (iters < 2 || iters % 2) return fail(PFDD_ERR_ARG, "iters must be even and >= 2");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!handles[i]) return fail(PFDD_ERR_ARG, "null handle %d", i);
+        if (handles[i]->device != handles[0]->device) return fail(PFDD_ERR_ARG, "handles on different devices");
+    }
+    pfdd_gl_graph *g = new pfdd_gl_graph();
+    g->hs.assign(handles, handles + n);
+    g->iters = iters;
+    g->ver.assign(n, 0);
+    g->cur.assign(n, -1);
+    hipError_t e = hipSetDevice(handles[0]->device);
+    if (e == hipSuccess) e = hipStreamCreate(&g->stream);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(PFDD_ERR_HIP, "graph stream: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return PFDD_OK;
+}
+
+int pfdd_gl_graph_run(pfdd_gl_graph *g, int nsteps, double tol, pfdd_gl_reduce *out)
+{
+    if (!g) return fail(PFDD_ERR_ARG, "null graph");
+    if (nsteps < 0) return fail(PFDD_ERR_ARG, "nsteps < 0");
+    HIPCHK(hipSetDevice(g->hs[0]->device));
+    const size_t n = g->hs.size();
+    bool stale = !g->exec || tol != g->tol;
+    for (size_t i = 0; i < n; ++i) stale |= g->hs[i]->version != g->ver[i] || g->hs[i]->cur != g->cur[i];
+    // order after each handle's earlier work
+    for (pfdd_gl *h : g->hs) HIPCHK(hipStreamSynchronize(h->stream));
+    int rc;
+    if (stale && (rc = graph_capture(g, tol))) return rc;
+    std::vector<int> cur0(n);
+    for (size_t i = 0; i < n; ++i) {
+        pfdd_gl *h = g->hs[i];
+        cur0[i] = h->cur;
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, g->stream, h->done, h->steps);
+        HIPCHK(hipGetLastError());
+    }
+    for (int r = 0; r < nsteps / g->iters; ++r) HIPCHK(hipGraphLaunch(g->exec, g->stream));
+    // the remainder eagerly, on the graph stream
+    const int rem = nsteps % g->iters;
+    for (int it = 0; it < rem; ++it)
+        for (pfdd_gl *h : g->hs) {
+            hipStream_t s0 = h->stream;
+            h->stream = g->stream;
+            rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
+            h->stream = s0;
+            if (rc) return rc;
+        }
+    HIPCHK(hipStreamSynchronize(g->stream));
+    int first_err = PFDD_OK;
+    for (size_t i = 0; i < n; ++i) {
+        pfdd_gl_reduce r{};
+        rc = collect_steps(g->hs[i], cur0[i], &r);
+        if (rc && !first_err) first_err = rc;
+        if (out) out[i] = r;
+    }
+    return first_err;
+}
+
+int pfdd_gl_graph_destroy(pfdd_gl_graph *g)
+{
+    if (!g) return PFDD_OK;
+    hipSetDevice(g->hs[0]->device);
+    if (g->stream) hipStreamSynchronize(g->stream);
+    graph_release(g);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
     return PFDD_OK;
 }
 

@@ -70,6 +70,7 @@ EXPORTS = [
     "pfdd_gl_set_tau", "pfdd_gl_set_core_coef", "pfdd_gl_step", "pfdd_gl_xi_sums",
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
+    "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
 ]
 
 _lib = None
@@ -107,6 +108,10 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_tape_record": (c_int, [vp, c_int, c_int, ctypes.POINTER(GLReduce)]),
         "pfdd_gl_tape_backward": (c_int, [vp, _dp, _dp, _dp, _dp]),
         "pfdd_gl_tape_backward_device": (c_int, [vp, vp, vp, _dp, _dp]),
+        "pfdd_gl_create_batch": (c_int, [ctypes.POINTER(GLDesc), c_int, ctypes.POINTER(vp)]),
+        "pfdd_gl_graph_create": (c_int, [ctypes.POINTER(vp), c_int, c_int, ctypes.POINTER(vp)]),
+        "pfdd_gl_graph_run": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
+        "pfdd_gl_graph_destroy": (c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -134,11 +139,13 @@ class GLSolver:
 
     def __init__(self, shape, xn, xb, four, C44, C12, C11, dslip, cddt, *, S12=0.0, S44=0.0,
                  core_model=CORE_NONE, coef=None, ncoef=0, amp=None, xo=None,
-                 precision=PFDD_FP64, device=0, stored_B=False):
+                 precision=PFDD_FP64, device=0, stored_B=False, batch=1):
         L = load_library()
         self.shape = tuple(int(v) for v in shape)
         self._xn, self._xb = _f64(xn).reshape(-1, 3), _f64(xb).reshape(-1, 3)
         self.S = self._xn.shape[0]
+        self.batch = int(batch)
+        self.SM = self.S * self.batch
         self.precision = precision
         self._coef = _f64(coef).reshape(-1) if coef is not None else None
         self._amp = _f64(amp).reshape(-1) if amp is not None else None
@@ -158,7 +165,7 @@ class GLSolver:
         d.stream = None
         d.flags = FLAG_STORED_B if stored_B else 0
         h = ctypes.c_void_p()
-        _check(L.pfdd_gl_create(ctypes.byref(d), ctypes.byref(h)))
+        _check(L.pfdd_gl_create_batch(ctypes.byref(d), self.batch, ctypes.byref(h)))
         self._h = h
         self._lib = L
 
@@ -182,11 +189,11 @@ class GLSolver:
     # ------------------------------------------------------------ field I/O
     def set_xi(self, xi):
         xi = np.ascontiguousarray(xi, dtype=np.complex128)
-        assert xi.shape == (self.S,) + self.shape, xi.shape
+        assert xi.shape == (self.SM,) + self.shape, xi.shape
         _check(self._lib.pfdd_gl_set_xi(self._h, xi.ctypes.data_as(_dp)))
 
     def get_xi(self):
-        out = np.empty((self.S,) + self.shape, dtype=np.complex128)
+        out = np.empty((self.SM,) + self.shape, dtype=np.complex128)
         _check(self._lib.pfdd_gl_get_xi(self._h, out.ctypes.data_as(_dp)))
         return out
 
@@ -202,8 +209,8 @@ class GLSolver:
         _check(self._lib.pfdd_gl_get_xi_device(self._h, ctypes.c_void_p(ptr)))
 
     def set_tau(self, tau):
-        t = _f64(tau)
-        assert t.size == self.S
+        t = _f64(tau).reshape(-1)
+        assert t.size == self.SM
         _check(self._lib.pfdd_gl_set_tau(self._h, _ptr(t)))
 
     def set_core_coef(self, coef, ncoef):
@@ -244,7 +251,7 @@ class GLSolver:
         return dtau, dcoef[: self.S * int(ncoef)].reshape(self.S, int(ncoef))
 
     def xi_sums(self):
-        out = np.zeros(self.S)
+        out = np.zeros(self.SM)
         _check(self._lib.pfdd_gl_xi_sums(self._h, _ptr(out)))
         return out
 
@@ -280,3 +287,31 @@ class GLSolver:
         tot = ctypes.c_double()
         _check(self._lib.pfdd_gl_traffic(self._h, _ptr(k), ctypes.byref(tot)))
         return k, tot.value
+
+
+class GLGraph:
+    """Several solvers stepped together by one hipGraph (pfdd_gl_graph_*)."""
+
+    def __init__(self, solvers, iters=2):
+        L = load_library()
+        self._solvers = list(solvers)  # keep the handles alive
+        arr = (ctypes.c_void_p * len(self._solvers))(*[s._h.value for s in self._solvers])
+        g = ctypes.c_void_p()
+        _check(L.pfdd_gl_graph_create(arr, len(self._solvers), int(iters), ctypes.byref(g)))
+        self._g, self._lib = g, L
+
+    def run(self, nsteps, tol=0.0):
+        out = (GLReduce * len(self._solvers))()
+        _check(self._lib.pfdd_gl_graph_run(self._g, int(nsteps), float(tol), out))
+        return [r.as_dict() for r in out]
+
+    def close(self):
+        if getattr(self, "_g", None):
+            self._lib.pfdd_gl_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -145,3 +145,61 @@ def fcc_loops_problem(shape, nloops=8, rmin=8, rmax=20, noise=1e-3, shear_pa=400
             "C44": m.C44, "C12": m.C12, "C11": m.C11, "S12": m.S12, "S44": m.S44, "dslip": m.dslip,
             "mu_s": m.mu_s, "cddt": cddt, "core_coef": core_coef(m, 3), "tau": tau, "sigma_rot": srot,
             "xi0": xi, "loops": loops, "material": m.as_dict()}
+
+
+def lognormal_amplitude(shape, corr_len=4.0, cv=0.15, seed=576):
+    """Per-cell GSF amplitude of an alloy realization: lognormal with mean 1 and
+    coefficient of variation `cv` of a unit-variance periodic Gaussian random
+    field whose covariance is exp(-r^2 / (2 corr_len^2)) (white noise filtered
+    in Fourier space by the square root of that covariance's spectrum,
+    exp(-pi^2 corr_len^2 |k|^2) with k in cycles per cell), standardised."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(tuple(shape))
+    k = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
+    filt = np.exp(-np.pi ** 2 * corr_len ** 2 * (k[0] ** 2 + k[1] ** 2 + k[2] ** 2))
+    g = np.fft.ifftn(np.fft.fftn(w) * filt).real
+    g = (g - g.mean()) / g.std()
+    s2 = np.log1p(cv * cv)
+    return np.exp(-0.5 * s2 + np.sqrt(s2) * g)
+
+
+def edge_dislocation_field(shape, S, a=0):
+    """Straight edge dislocation pair on the slip plane k = nz/2: Re xi_a = 1 for
+    i < nx/2 (the periodic box holds the two boundary lines x = 0 and x = nx/2)."""
+    nx, ny, nz = shape
+    xi = np.zeros((S,) + tuple(shape), dtype=np.complex128)
+    xi[a, : nx // 2, :, nz // 2] = 1.0
+    return xi
+
+
+def alloy_ensemble(groups=((48, 16), (64, 24), (96, 8)), stress_pa=(200e6, 600e6), corr_len=4.0, cv=0.15,
+                   seed=576, cddt=0.01):
+    """Batched alloy ensemble: for each (n, count) group, `count` realizations on
+    an n^3 grid with the three in-plane systems of basal3 (n = z; b at 0, 120,
+    240 deg), Cu-like material, sine core scaled per cell by a lognormal
+    amplitude field, a straight edge dislocation of system 0, and an applied
+    shear sigma_i ~ U(stress_pa) acting on system 0.  Returns one dict per
+    group with the concatenated member inputs of a batch handle."""
+    m = cu_like()
+    sl = slip("basal3")
+    S = sl["xn"].shape[0]
+    b0 = sl["xb"][0] / np.linalg.norm(sl["xb"][0])
+    n0 = sl["xn"][0] / np.linalg.norm(sl["xn"][0])
+    unit = np.outer(b0, n0) + np.outer(n0, b0)
+    rng = np.random.default_rng(seed)
+    coef = np.tile(core_coef(m, 3), S)
+    out = []
+    member = 0
+    for n, count in groups:
+        shape = (n, n, n)
+        xi0 = np.concatenate([edge_dislocation_field(shape, S) for _ in range(count)])
+        stresses = rng.uniform(stress_pa[0], stress_pa[1], size=count)
+        tau = np.concatenate([resolved_shear("basal3", (sp / m.mu) * unit, m.dslip)[0] for sp in stresses])
+        amp = np.concatenate([np.broadcast_to(lognormal_amplitude(shape, corr_len, cv, seed + 1000 + member + q),
+                                              (S,) + shape) for q in range(count)])
+        member += count
+        out.append({"shape": shape, "batch": count, "S": S, "xn": sl["xn"], "xb": sl["xb"], "four": sl["four"],
+                    "C44": m.C44, "C12": m.C12, "C11": m.C11, "S12": m.S12, "S44": m.S44, "dslip": m.dslip,
+                    "mu": m.mu, "cddt": cddt, "coef": coef, "ncoef": 2, "tau": tau, "stress_pa": stresses,
+                    "xi0": xi0, "amp": np.ascontiguousarray(amp).reshape(-1)})
+    return out

@@ -212,17 +212,16 @@ def test_torch_autograd_function():
     nsteps, every = 15, 4
     target = target_of(case)
     s = gpu_case_solver(case)
-    dev = torch.device("cuda:0")
     # tau = sigma * r: resolved shear of a scalar shear stress (linear, as resolSS_Schmid)
-    r = torch.tensor(np.asarray(case["tau"]) / 0.005, device=dev)
+    r = torch.tensor(np.asarray(case["tau"]) / 0.005)
     coef0 = case["core_par"].reshape(case["S"], case["nm"])
 
-    def loss_of(sig_val, xi0_np=None):
-        xi0 = torch.tensor(case["xi0"] if xi0_np is None else xi0_np, device=dev, requires_grad=True)
-        sig = torch.tensor(sig_val, dtype=torch.float64, device=dev, requires_grad=True)
-        coef = torch.tensor(coef0, device=dev, requires_grad=True)
+    def loss_of(sig_val):
+        xi0 = torch.tensor(case["xi0"], requires_grad=True)
+        sig = torch.tensor(sig_val, dtype=torch.float64, requires_grad=True)
+        coef = torch.tensor(coef0, requires_grad=True)
         xT = pfdd_torch.gl_rollout(s, xi0, sig * r, coef, nsteps, every)
-        L = (xT - torch.tensor(target, device=dev)).abs().pow(2).sum()
+        L = (xT - torch.tensor(target)).abs().pow(2).sum()
         return L, xi0, sig, coef
 
     L, xi0, sig, coef = loss_of(0.005)
@@ -237,11 +236,11 @@ def test_torch_autograd_function():
     # same numbers as the plain C ABI reverse pass
     s.set_core_coef(coef0.reshape(-1), case["nm"])
     s.set_xi(case["xi0"])
-    s.set_tau(0.005 * r.cpu().numpy())
+    s.set_tau(0.005 * r.numpy())
     s.tape_record(nsteps, every)
     xT = s.get_xi()
     lam, dtau, dc = s.tape_backward(2.0 * (xT - target), case["nm"])
     s.close()
-    assert np.allclose(xi0.grad.cpu().numpy(), lam, rtol=1e-12, atol=1e-12 * np.max(np.abs(lam)))
-    assert np.allclose(coef.grad.cpu().numpy(), dc, rtol=1e-12, atol=1e-14)
-    assert abs(g - float(np.dot(dtau, r.cpu().numpy()))) <= 1e-10 * abs(g)
+    assert np.allclose(xi0.grad.numpy(), lam, rtol=1e-12, atol=1e-12 * np.max(np.abs(lam)))
+    assert np.allclose(coef.grad.numpy(), dc, rtol=1e-12, atol=1e-14)
+    assert abs(g - float(np.dot(dtau, r.numpy()))) <= 1e-10 * abs(g)

@@ -1,0 +1,120 @@
+"""Batched runs (pfdd_gl_create_batch) and hipGraph replay (pfdd_gl_graph_*).
+
+A batch member runs exactly the arithmetic of a single-run handle, so its
+field must equal, bit for bit, that of a separate handle given the same
+inputs; a graph replay must equal the same iterations launched eagerly.
+The single-run handles themselves are pinned against the oracle by
+test_gpu_parity.py.
+"""
+import numpy as np
+import pytest
+
+import problems as P
+
+
+def member_cases(shape, n, masked=False):
+    cases = []
+    nx, ny, nz = shape
+    for m in range(n):
+        xo = None
+        if masked:
+            xo = np.zeros((3,) + tuple(shape))
+            xo[m % 3, (m + 1) * nx // 5, :, nz // 2] = 1.0
+        c = P.make_case(shape=shape, noise=1e-2, sigma_mu=0.002 + 0.002 * m, seed=576 + 17 * m,
+                        loops=[(m % 3, 2, nz // 2, nx / 2 - m, ny / 2 + m, max(2, nx // 5))], xo=xo)
+        cases.append(c)
+    return cases
+
+
+def batch_solver(cases, precision=0):
+    import pfdd_gl
+    c0, m = cases[0], cases[0]["mat"]
+    xo = None
+    if any(c["xo"] is not None for c in cases):
+        xo = np.concatenate([c["xo"] if c["xo"] is not None else np.zeros((c["S"],) + c["shape"])
+                             for c in cases])
+    s = pfdd_gl.GLSolver(c0["shape"], c0["xn"], c0["xb"], c0["geo"]["four"], m["C44"], m["C12"], m["C11"],
+                         m["dslip"], c0["cddt"], S12=m["S12"], S44=m["S44"], xo=xo, precision=precision,
+                         batch=len(cases), **c0["gpu_core"])
+    s.set_xi(np.concatenate([c["xi0"] for c in cases]))
+    s.set_tau(np.concatenate([c["tau"] for c in cases]))
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,masked", [(0, False), (1, False), (0, True)])
+def test_batch_members_equal_single_runs(precision, masked):
+    print(f"case: batch of 3 x (16^3 x 3) vs single handles, precision {precision}, masked {masked}",
+          flush=True)
+    cases = member_cases((16, 16, 16), 3, masked)
+    b = batch_solver(cases, precision)
+    b.step(25)
+    xb = b.get_xi()
+    sums = b.xi_sums()
+    b.close()
+    for m, c in enumerate(cases):
+        s = P.gpu_solver(c, precision=precision)
+        s.step(25)
+        x = s.get_xi()
+        ss = s.xi_sums()
+        s.close()
+        assert np.array_equal(xb[3 * m:3 * m + 3], x), m
+        assert np.allclose(sums[3 * m:3 * m + 3], ss, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager_steps():
+    print("case: hipGraph over batch 2x16^3, batch 3x32^3 and a single 16^3 handle; 21 + 10 + 6 steps",
+          flush=True)
+    import pfdd_gl
+    ca = member_cases((16, 16, 16), 2)
+    cb = member_cases((32, 32, 32), 3)
+    cc = member_cases((16, 16, 16), 1)[0]
+
+    def make():
+        return [batch_solver(ca), batch_solver(cb, precision=1), P.gpu_solver(cc)]
+
+    eager, graphed = make(), make()
+    g = pfdd_gl.GLGraph(graphed, iters=4)
+    res = g.run(21)
+    for s in eager:
+        s.step(21)
+    for s, t, r in zip(eager, graphed, res):
+        assert r["steps_done"] == 21
+        assert np.array_equal(s.get_xi(), t.get_xi())
+    # new stresses: device tau for the batches (no new capture), kernel
+    # argument for the single handle (new capture)
+    new_a = np.concatenate([c["tau"] for c in ca]) * 1.5
+    new_c = np.asarray(cc["tau"]) * -2.0
+    for grp in (eager, graphed):
+        grp[0].set_tau(new_a)
+        grp[2].set_tau(new_c)
+    g.run(10)
+    for s in eager:
+        s.step(10)
+    for s, t in zip(eager, graphed):
+        assert np.array_equal(s.get_xi(), t.get_xi())
+    # an odd number of eager steps between graph runs
+    for grp in (eager, graphed):
+        grp[1].step(1)
+    g.run(6)
+    for s in eager:
+        s.step(6)
+    for s, t in zip(eager, graphed):
+        assert np.array_equal(s.get_xi(), t.get_xi())
+    g.close()
+    for s in eager + graphed:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_single_run_only_calls():
+    print("case: batch handle error paths", flush=True)
+    import pfdd_gl
+    cases = member_cases((16, 16, 16), 2)
+    b = batch_solver(cases)
+    with pytest.raises(pfdd_gl.PfddError, match="UNSUPPORTED"):
+        b.diag(np.zeros((3, 3)))
+    with pytest.raises(pfdd_gl.PfddError, match="UNSUPPORTED"):
+        b.tape_record(2, 1)
+    b.close()
