@@ -205,6 +205,234 @@ def cpu_baseline(prob, nsteps=3):
                       f"FP64 C oracle with {cores} OpenMP threads"}
 
 
+def bench_ensemble(args):
+    """Batched alloy ensemble: 16 x 48^3 + 24 x 64^3 + 8 x 96^3 realizations, 3 systems
+    each, FP32, per-cell lognormal GSF amplitude, per-realization shear in
+    [200, 600] MPa, straight edge dislocation; every GL iteration of all 48
+    realizations is one hipGraph (3 batch handles) replayed back to back."""
+    groups = setup.alloy_ensemble()
+    prec = pfdd_gl.PFDD_FP64 if args.precision == "fp64" else pfdd_gl.PFDD_FP32
+    solvers = []
+    for g in groups:
+        s = pfdd_gl.GLSolver(g["shape"], g["xn"], g["xb"], g["four"], g["C44"], g["C12"], g["C11"], g["dslip"],
+                             g["cddt"], S12=g["S12"], S44=g["S44"], core_model=pfdd_gl.CORE_SERIES,
+                             coef=g["coef"], ncoef=g["ncoef"], amp=g["amp"], precision=prec, batch=g["batch"])
+        s.set_xi(g["xi0"])
+        s.set_tau(g["tau"])
+        solvers.append(s)
+    graph = pfdd_gl.GLGraph(solvers, iters=10)
+    graph.run(args.warmup - args.warmup % 10 or 10)   # capture + warm-up (even count: same buffers)
+    for s, g in zip(solvers, groups):
+        s.set_xi(g["xi0"])
+    nsteps = args.steps
+    half = nsteps // 2
+    # two timed halves; the per-system sums of Re xi between them give each
+    # realization's glide over the second half (after the initial core relaxation)
+    t0 = time.perf_counter()
+    graph.run(nsteps - half)
+    dt = time.perf_counter() - t0
+    mid = [s.xi_sums() for s in solvers]
+    t0 = time.perf_counter()
+    res = graph.run(half)
+    dt += time.perf_counter() - t0
+    end = [s.xi_sums() for s in solvers]
+    nreal = sum(g["batch"] for g in groups)
+    step_bytes = sum(s.traffic()[1] for s in solvers)
+    members = []
+    for g, a, b in zip(groups, mid, end):
+        n = g["shape"][0]
+        for q in range(g["batch"]):
+            # swept area of system 0 / plane area; two dislocation lines per box
+            glide = (b[q * g["S"]] - a[q * g["S"]]) / (n * n) * n / 2.0
+            members.append({"grid": n, "stress_MPa": float(g["stress_pa"][q] / 1e6),
+                            "velocity_cells_per_step": glide / half})
+    # disorder-free reference crystal (amplitude 1) at 400 MPa, same protocol
+    g0 = setup.alloy_ensemble(groups=((64, 1),), stress_pa=(400e6, 400e6))[0]
+    ref = pfdd_gl.GLSolver(g0["shape"], g0["xn"], g0["xb"], g0["four"], g0["C44"], g0["C12"], g0["C11"],
+                           g0["dslip"], g0["cddt"], core_model=pfdd_gl.CORE_SERIES, coef=g0["coef"],
+                           ncoef=g0["ncoef"], precision=prec)
+    ref.set_xi(g0["xi0"])
+    ref.set_tau(g0["tau"])
+    ref.step(nsteps - half)
+    a0 = ref.xi_sums()[0]
+    ref.step(half)
+    v0 = (ref.xi_sums()[0] - a0) / (64 * 64) * 32.0 / half / 400.0  # cells per step per MPa
+    ref.close()
+    for mbr in members:
+        mbr["critical_stress_MPa_est"] = (mbr["stress_MPa"] - mbr["velocity_cells_per_step"] / v0
+                                          if v0 > 0 else None)
+    for s in solvers:
+        s.close()
+    graph.close()
+    line = {
+        "metric": "realization GL steps/s, batched alloy ensemble in one hipGraph",
+        "value": nreal * nsteps / dt, "unit": "realization-steps/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / nsteps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic (lognormal GSF amplitude fields, uniform stresses)",
+        "config": {"workload": "16 x 48^3 + 24 x 64^3 + 8 x 96^3 realizations x 3 basal systems, sine core "
+                               "x per-cell lognormal amplitude (mean 1, CV 0.15, correlation length 4), "
+                               "shear U[200,600] MPa, straight edge dislocation, 10 GL iterations per graph launch",
+                   "realizations": nreal},
+        "hbm": {"bytes_per_step": step_bytes, "achieved_GBps": step_bytes * nsteps / dt / 1e9,
+                "frac": step_bytes * nsteps / dt / PEAK_HBM},
+        "steps_done": [r["steps_done"] for r in res],
+        "critical_stress_estimator": "applied stress minus (glide velocity over the second half of the run) / "
+                                     "(velocity per MPa of the disorder-free 64^3 crystal at 400 MPa)",
+        "reference_velocity_cells_per_step_per_MPa": v0,
+        "realizations": members,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_adjoint(args):
+    """Reverse mode on the 64^3 x 3 loop problem: 500 GL iterations forward with
+    checkpoints every 50, then the adjoint sweep; cost ratio and a central
+    finite-difference check of dL/dcoef and dL/dsigma (h = 1e-6)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    n, nsteps, every = 64, args.steps, 50
+    m = setup.cu_like()
+    sl = setup.slip("basal3")
+    S = 3
+    rng = np.random.default_rng(576)
+    base = setup.core_coef(m, 3)
+    c_true = np.zeros((S, 4))
+    c_true[:, 0], c_true[:, 1] = base[0], base[1]
+    c_true[:, 2] = 0.1 * base[0]
+    c_true[:, 3] = -0.05 * base[0]
+    c_true *= 1.0 + 0.2 * (2.0 * rng.random((S, 4)) - 1.0)
+    c_guess = c_true.copy()
+    c_guess[:, 0], c_guess[:, 1] = base[0], base[1]
+    c_guess[:, 2], c_guess[:, 3] = 0.1 * base[0], -0.05 * base[0]
+    b0 = sl["xb"][0] / np.linalg.norm(sl["xb"][0])
+    n0 = sl["xn"][0] / np.linalg.norm(sl["xn"][0])
+    unit = np.outer(b0, n0) + np.outer(n0, b0)
+    sig_mu = 250e6 / m.mu
+
+    def tau_of(sig):
+        return setup.resolved_shear("basal3", sig * unit, m.dslip)[0]
+
+    x0 = np.zeros((S, n, n, n), dtype=np.complex128)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x0[0, :, :, n // 2][(ii - n / 2) ** 2 + (jj - n / 2) ** 2 <= 144] = 1.0
+    x0 += 2e-3 * (setup.ranmar(576, x0.size).reshape(x0.shape) - 0.5)
+    out = {}
+    for prec, tag in ((pfdd_gl.PFDD_FP64, "fp64"), (pfdd_gl.PFDD_FP32, "fp32")):
+        s = pfdd_gl.GLSolver((n, n, n), sl["xn"], sl["xb"], sl["four"], m.C44, m.C12, m.C11, m.dslip, 0.01,
+                             core_model=pfdd_gl.CORE_SERIES, coef=c_true.reshape(-1), ncoef=4, precision=prec)
+
+        def forward(c, sig, record=False):
+            s.set_core_coef(c.reshape(-1), 4)
+            s.set_tau(tau_of(sig))
+            s.set_xi(x0)
+            if record:
+                s.tape_record(nsteps, every)
+            else:
+                s.step(nsteps)
+            return s.get_xi()
+
+        target = forward(c_true, sig_mu)
+        loss = lambda x: float(np.sum(np.abs(x - target) ** 2))
+        forward(c_guess, sig_mu)                       # warm-up
+        s.set_xi(x0)
+        s.step(0)
+        t0 = time.perf_counter(); s.step(nsteps); t_fwd = time.perf_counter() - t0
+        s.set_core_coef(c_guess.reshape(-1), 4)
+        s.set_tau(tau_of(sig_mu))
+        s.set_xi(x0)
+        t0 = time.perf_counter(); s.tape_record(nsteps, every); t_rec = time.perf_counter() - t0
+        xT = s.get_xi()
+        t0 = time.perf_counter()
+        dxi0, dtau, dcoef = s.tape_backward(2.0 * (xT - target), 4)
+        t_bwd = time.perf_counter() - t0
+        dsig = float(np.dot(dtau, tau_of(1.0)))
+        res = {"forward_ms_per_step": 1e3 * t_fwd / nsteps, "record_ms_per_step": 1e3 * t_rec / nsteps,
+               "backward_ms_per_step": 1e3 * t_bwd / nsteps, "adjoint_cost_vs_forward": t_bwd / t_fwd,
+               "loss": loss(xT)}
+        if prec == pfdd_gl.PFDD_FP64:
+            h, worst, fds, adj = 1e-6, 0.0, [], []
+            for a in range(S):
+                for q in range(4):
+                    cp, cm = c_guess.copy(), c_guess.copy()
+                    cp[a, q] += h
+                    cm[a, q] -= h
+                    fd = (loss(forward(cp, sig_mu)) - loss(forward(cm, sig_mu))) / (2 * h)
+                    worst = max(worst, abs(fd - dcoef[a, q]) / max(abs(fd), 1e-30))
+                    fds.append(fd)
+                    adj.append(dcoef[a, q])
+            fd = (loss(forward(c_guess, sig_mu + h)) - loss(forward(c_guess, sig_mu - h))) / (2 * h)
+            worst = max(worst, abs(fd - dsig) / abs(fd))
+            fds.append(fd)
+            adj.append(dsig)
+
+            def normwise(x, y):
+                x, y = np.asarray(x), np.asarray(y)
+                return float(np.linalg.norm(x - y) / np.linalg.norm(x))
+            res["fd_check"] = {"h": h, "params": "12 GSF coefficients + applied stress",
+                               "max_rel_diff_adjoint_vs_central_fd": worst,
+                               "normwise_rel_diff": normwise(fds, adj)}
+            out["grad64"] = (dcoef.copy(), dsig)
+        else:
+            g64, s64 = out["grad64"]
+            res["rel_err_grad_vs_fp64"] = float(max(np.max(np.abs(dcoef - g64)) / np.max(np.abs(g64)),
+                                                    abs(dsig - s64) / abs(s64)))
+        out[tag] = res
+        s.close()
+    out.pop("grad64")
+    line = {"metric": "adjoint cost relative to a forward GL step, 64^3 x 3, 500 steps, checkpoints every 50",
+            "value": out["fp64"]["adjoint_cost_vs_forward"], "unit": "x forward", "n_gpus": 1,
+            "steps": nsteps, "warmup": 0, "higher_is_better": False, "dtype": "f64",
+            "data": "synthetic (loop r=12 + noise, hidden GSF coefficients +-20%)",
+            "config": {"workload": "64^3 x 3 basal systems, 4-term sine-series core, 250 MPa, L2 loss vs target"},
+            "fp64": out["fp64"], "fp32": out["fp32"]}
+    print(json.dumps(line))
+    return 0
+
+
+def bench_frank_read(args):
+    """512 x 512 x 256 skewed grid, 3 systems, pinned Frank-Read source, 500 MPa:
+    FP32 and FP64 runs of the same length, steps/s, HBM rate and FP32 error."""
+    prob = setup.frank_read_skewed()
+    out = {}
+    xs = {}
+    for prec, tag in ((pfdd_gl.PFDD_FP32, "fp32"), (pfdd_gl.PFDD_FP64, "fp64")):
+        s = pfdd_gl.GLSolver(prob["shape"], prob["xn"], prob["xb"], prob["four"], prob["C44"], prob["C12"],
+                             prob["C11"], prob["dslip"], prob["cddt"], S12=prob["S12"], S44=prob["S44"],
+                             core_model=pfdd_gl.CORE_SINE, coef=prob["core_coef"], xo=prob["xo"],
+                             precision=prec)
+        s.set_xi(prob["xi0"])
+        s.set_tau(prob["tau"])
+        s.step(0)
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.steps:
+            k = min(500, args.steps - done)
+            r = s.step(k)
+            done += k
+            print(f"frank-read {tag}: {done} steps", file=sys.stderr, flush=True)
+        dt = time.perf_counter() - t0
+        step_bytes = s.traffic()[1]
+        out[tag] = {"steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                    "hbm_GBps": step_bytes * args.steps / dt / 1e9, "bytes_per_step": step_bytes,
+                    "e_total": r["e_total"], "xiave": r["xiave"]}
+        xs[tag] = s.get_xi()
+        s.close()
+    err = float(np.max(np.abs(xs["fp32"] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
+    e_err = abs(out["fp32"]["e_total"] - out["fp64"]["e_total"]) / abs(out["fp64"]["e_total"])
+    line = {"metric": "GL steps/s, 512x512x256 skewed grid x 3 systems, Frank-Read source",
+            "value": out["fp32"]["steps_per_s"], "unit": "GL steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 0, "ms_per_step": out["fp32"]["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (pinned straight segment of 64 cells)",
+            "config": {"workload": "512x512x256, primitive y = [1/2 sqrt3/2 0], basal3 systems, sine core, "
+                                   "500 MPa shear on system 0, obstacle mask pins both segment ends"},
+            "fp32": out["fp32"], "fp64": out["fp64"], "rel_err_xi_fp32_vs_fp64": err,
+            "rel_err_energy_fp32_vs_fp64": e_err}
+    print(json.dumps(line))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +446,9 @@ def main():
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="turn two rocprofv3 --pmc outputs into pmc_traffic_*.json")
     ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles"))
+    ap.add_argument("--workload", default="c1", choices=["c1", "ensemble", "adjoint", "frank-read"],
+                    help="c1: the 128^3 x 12 headline (default); the others print their own JSON line")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="ensemble precision")
     args = ap.parse_args()
     if args.pmc_probe:
         return pmc_probe(args.size)
@@ -227,6 +458,13 @@ def main():
         print(f"bench.py: this build runs the GL solver on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design note, not built.", file=sys.stderr)
         return 2
+
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
+    if args.workload == "frank-read":
+        return bench_frank_read(args)
 
     prob = make_workload(args.size)
     out = {}

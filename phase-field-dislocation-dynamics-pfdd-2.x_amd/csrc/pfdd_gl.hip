@@ -507,11 +507,12 @@ __global__ void __launch_bounds__(NTHR)
     }
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
-    // the old field of this thread's output cells, fetched while the FFT runs
-    // (8-byte elements only: batching 16-byte loads measured slower)
+    // the old field of this thread's output cells, all loads in flight at
+    // once: 8-byte elements while the FFT runs, 16-byte elements right after
+    // it (holding them in registers across the FFT measured slower)
     constexpr int PER = (TOT + NTHR - 1) / NTHR;
     constexpr bool PREF = sizeof(V) < 16;
-    V xold[PREF ? PER : 1];
+    V xold[PER];
     if constexpr (PREF) {
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
@@ -522,11 +523,19 @@ __global__ void __launch_bounds__(NTHR)
     }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+    if constexpr (!PREF) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int e = tid + t * NTHR;
+            const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
+            xold[t] = b >= 0 ? xo[b + e % N] : mk<V>((R)0, (R)0);
+        }
+    }
 
     const R cddt = (R)P.cddt, invN = (R)P.invN;
     const int nm = P.ncoef;
     double s_norm = 0.0, s_ave = 0.0, s_core = 0.0, s_str = 0.0;
-#pragma unroll(PREF ? PER : 2)
+#pragma unroll
     for (int t = 0; t < PER; ++t) {
         const int e = tid + t * NTHR;
         if (TOT % NTHR != 0 && e >= TOT) continue;
@@ -537,8 +546,7 @@ __global__ void __launch_bounds__(NTHR)
         const int a = lsys[l], as = lcs[l];
         const V g = lds.ld(p * LDP + l);
         V x;
-        if constexpr (PREF) x = xold[t];
-        else x = xo[idx];
+        x = xold[t];
         // core energy density and its derivative at Re xi (sine series)
         R sp, cp;
         if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);

@@ -5,6 +5,7 @@ This is the Python face of the same host layer the deck driver uses."""
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -203,3 +204,34 @@ def alloy_ensemble(groups=((48, 16), (64, 24), (96, 8)), stress_pa=(200e6, 600e6
                     "mu": m.mu, "cddt": cddt, "coef": coef, "ncoef": 2, "tau": tau, "stress_pa": stresses,
                     "xi0": xi0, "amp": np.ascontiguousarray(amp).reshape(-1)})
     return out
+
+
+def frank_read_skewed(shape=(512, 512, 256), seg=64, shear_pa=500e6, cddt=0.01):
+    """Large single-grid run: Frank-Read source on a 60-degree skewed grid
+    (primitive x = [1 0 0], y = [1/2 sqrt3/2 0], z = [0 0 1]), the three basal3
+    systems (n = z, b at 0/120/240 deg in the lattice frame, rotated by the
+    reciprocal matrix as set_slip does), Cu-like material, sine core.
+    Initial field: Re xi_0 = 1 on the slip plane k = nz/2 for the `seg` cells
+    i in [nx/2 - seg/2, nx/2 + seg/2) and the two rows j = ny/2 - 2, ny/2 - 1,
+    so the row boundary at j = ny/2 is a straight segment of `seg` cells.
+    Its ends are pinned by frozen cells (mask xo = 1, all systems): a 2 x 4
+    block at each end, i = nx/2 -+ (seg/2 + 1..2), j = ny/2 - 2 .. ny/2 + 1.
+    Applied shear sigma = tau0 (b0 n0 + n0 b0) on system 0 with tau0 = shear_pa."""
+    nx, ny, nz = shape
+    prim = ([1.0, 0.0, 0.0], [0.5, math.sqrt(3.0) / 2.0, 0.0], [0.0, 0.0, 1.0])
+    m = cu_like()
+    sl = slip("basal3", prim=prim)
+    S = sl["xn"].shape[0]
+    b0 = sl["xb"][0] / np.linalg.norm(sl["xb"][0])
+    n0 = sl["xn"][0] / np.linalg.norm(sl["xn"][0])
+    sigma = (shear_pa / m.mu) * (np.outer(b0, n0) + np.outer(n0, b0))
+    tau, srot = resolved_shear("basal3", sigma, m.dslip, prim=prim)
+    xi = np.zeros((S, nx, ny, nz), dtype=np.complex128)
+    i0, i1 = nx // 2 - seg // 2, nx // 2 + seg // 2
+    xi[0, i0:i1, ny // 2 - 2:ny // 2, nz // 2] = 1.0
+    xo = np.zeros((S, nx, ny, nz))
+    for a, b in ((i0 - 2, i0), (i1, i1 + 2)):
+        xo[:, a:b, ny // 2 - 2:ny // 2 + 2, nz // 2] = 1.0
+    return {"shape": tuple(shape), "xn": sl["xn"], "xb": sl["xb"], "four": sl["four"], "C44": m.C44,
+            "C12": m.C12, "C11": m.C11, "S12": m.S12, "S44": m.S44, "dslip": m.dslip, "cddt": cddt,
+            "core_coef": core_coef(m, 3), "tau": tau, "sigma_rot": srot, "xi0": xi, "xo": xo}
