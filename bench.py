@@ -43,7 +43,7 @@ def solver_for(prob, precision):
     s = pfdd_gl.GLSolver(prob["shape"], prob["xn"], prob["xb"], prob["four"], prob["C44"],
                          prob["C12"], prob["C11"], prob["dslip"], prob["cddt"], S12=prob["S12"],
                          S44=prob["S44"], core_model=pfdd_gl.CORE_SINE, coef=prob["core_coef"],
-                         precision=precision)
+                         precision=precision, autotune=True)
     s.set_xi(prob["xi0"])
     s.set_tau(prob["tau"])
     return s
@@ -474,6 +474,7 @@ def main():
         xi_end = s.get_xi() if tag else None
         roof = roofline(s)
         roof["traffic"] = load_pmc_traffic(tag, roof["kernel"])
+        roof["tiles"] = s.autotune()  # cached choice of the create-time tuning
         out[tag] = {"sec": dt, "steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                     "roofline": roof, "e_total": r["e_total"], "xi_end": xi_end}
         s.close()
@@ -517,6 +518,8 @@ def main():
             "fp32": {"bytes": f32["roofline"]["step_bytes"],
                      "frac": f32["roofline"]["step_bytes"] / (f32["ms_per_step"] * 1e-3) / PEAK_HBM}},
         "per_kernel": {"fp64": f64["roofline"]["per_kernel"], "fp32": f32["roofline"]["per_kernel"]},
+        "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update"],
+                            "fp64": f64["roofline"]["tiles"], "fp32": f32["roofline"]["tiles"]},
     }
     print(json.dumps(line))
     return 0

@@ -30,6 +30,7 @@ extern "C" {
 #define PFDD_GL_FLAG_STORED_B 1 /* build packed B(k) once and stream it every step (the
                                    reference's BB array) instead of evaluating the Green's
                                    function on the fly inside the x pass */
+#define PFDD_GL_FLAG_AUTOTUNE 2 /* run pfdd_gl_autotune(h, 4, NULL) at the end of create */
 
 enum pfdd_gl_status {
     PFDD_OK = 0,
@@ -183,7 +184,8 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
  * more fields) and running the adjoint iteration backwards:
  *   dxi_0[2*S*N]        dL/dxi_0 (may be NULL)
  *   dtau[S]             dL/dtau_a (may be NULL)
- *   dcoef[S*ncoef]      dL/dc_{a,m} of the core-energy coefficients in their
+ *   dcoef[S*ncoef]      (ncoef of the handle's core model, <= 8)
+ *                       dL/dc_{a,m} of the core-energy coefficients in their
  *                       sine-series form dE = A sum_m c_m m pi sin(2 m pi x)
  *                       (PERFECT: m = 1 -> An'; SINE: m = 1, 2 -> isf', Cn';
  *                       per system -- sum over a for a shared coefficient)
@@ -193,6 +195,15 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
 int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out);
 int pfdd_gl_tape_backward(pfdd_gl *h, const double *dxi_T, double *dxi_0, double *dtau, double *dcoef);
 int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, double *dtau, double *dcoef);
+
+/* Autotuner (no counterpart in the reference): times each candidate LDS tile
+ * shape (0, 1, 2: about 1K/2K/4K FP64 or 2K/4K/8K FP32 elements per tile) of
+ * the four line passes -- forward z, forward y, inverse y, inverse z + update
+ * -- over `nsteps` GL iterations on this handle's own buffers, and keeps the
+ * fastest per pass (choice[4], may be NULL).  The field and step counters are
+ * restored; the choice is cached for the process per grid, system count,
+ * batch, precision and options.  Results (xi) do not depend on the choice. */
+int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[4]);
 
 /* Average device time (ms) of each kernel of one GL iteration over `nsteps`
  * iterations, measured with HIP events on the handle's stream.  ms[0..4] =

@@ -152,13 +152,20 @@ constexpr int pick_radix(int m)
 
 // Lines per LDS tile for a transform of length N: a power of two giving about
 // 32 KiB of complex data per tile.
-template <typename R, int N> constexpr int tile_lines()
+// Lines per LDS tile of a line pass: the largest power of two whose tile
+// holds at most (1024 << TQ) FP64 or (2048 << TQ) FP32 elements, at least one
+// line, and whose padded tile fits 60 KiB of LDS.  TQ is the tile-shape
+// choice of the autotuner (0, 1, 2).
+template <typename R, int N, int TQ> constexpr int tile_lines_q()
 {
-    constexpr int target = sizeof(R) == 8 ? 2048 : 4096;
+    constexpr int target = (sizeof(R) == 8 ? 1024 : 2048) << TQ;
     int nl = 1;
     while (nl * 2 * N <= target) nl *= 2;
+    while (nl > 1 && (long long)N * (nl + 1) * 2 * (int)sizeof(R) > 61440) nl /= 2;
     return nl;
 }
+
+template <typename R, int N> constexpr int tile_lines() { return tile_lines_q<R, N, 1>(); }
 
 // One Stockham (auto-sort) radix-RAD stage over all NL lines of the tile,
 // in place in LDS: every thread reads all its inputs, barrier, writes.

@@ -26,8 +26,12 @@
 // init_loop :1628-1666 (no longer needed), Bmatrix :466-638, frec :280-367.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdarg>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -149,7 +153,7 @@ __global__ void __launch_bounds__(NTHR) k_bbuild(RB *__restrict__ B, BParams P, 
 }
 
 // ------------------------------------------------- generic FFT pass (1, 2, 4)
-template <typename R, int N, bool INV>
+template <typename R, int N, bool INV, int TQ>
 __global__ void __launch_bounds__(NTHR)
     k_fft_pass(const typename cvec<R>::type *__restrict__ in, typename cvec<R>::type *__restrict__ out,
                LineMap mi, LineMap mo, long long nlines, const typename cvec<R>::type *__restrict__ tw,
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(NTHR)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     const int tid = threadIdx.x;
@@ -479,18 +483,18 @@ __global__ void __launch_bounds__(NTHR)
 // does (it reads data_fftw = xi before the update).  Masked cells keep their
 // value; their xinorm term uses the previous unmasked cell's old value in
 // (a, i, j, k) order (prev[] precomputed on the host, -1 = the initial 0).
-template <typename R, typename RA, int N>
+template <typename R, int N, int TQ>
 __global__ void __launch_bounds__(NTHR)
     k_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ xo,
              typename cvec<R>::type *__restrict__ xn, const uint8_t *__restrict__ mask,
-             const long long *__restrict__ prev, const RA *__restrict__ amp, LineMap mi, LineMap mo,
+             const long long *__restrict__ prev, const R *__restrict__ amp, LineMap mi, LineMap mo,
              long long nlines, long long lines_per_sys, StepParams P,
              const typename cvec<R>::type *__restrict__ tw, double *__restrict__ partial,
              long long pstride, const int *__restrict__ done)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     __shared__ int lsys[NL], lcs[NL]; // system of the line in the batch / within its member
@@ -611,18 +615,18 @@ __global__ void __launch_bounds__(NTHR)
 // the update above, and leaves per-block sums of Re mu (slot 0) and of
 // Re mu A m pi sin(2 m pi x) (slot m) in gpart.  Every block covers lines of
 // one system (the host checks nx*ny % NL == 0).
-template <typename R, typename RA, int N>
+template <typename R, int N, int TQ>
 __global__ void __launch_bounds__(NTHR)
     k_adj_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ lam,
                  const typename cvec<R>::type *__restrict__ xs, typename cvec<R>::type *__restrict__ lam_out,
-                 const uint8_t *__restrict__ mask, const RA *__restrict__ amp, LineMap mi, LineMap mo,
+                 const uint8_t *__restrict__ mask, const R *__restrict__ amp, LineMap mi, LineMap mo,
                  long long nlines, long long lines_per_sys, StepParams P,
                  const typename cvec<R>::type *__restrict__ tw, double *__restrict__ gpart, long long gstride,
                  const int *__restrict__ done)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines<R, N>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     constexpr int PER = (TOT + NTHR - 1) / NTHR;
     constexpr int NG = 1 + MAXM;
     PFDD_TILE(V, lds, N * LDP);
@@ -936,68 +940,88 @@ bool size_supported(int n)
 bool systems_supported(int s) { return s == 1 || s == 2 || s == 3 || s == 4 || s == 6 || s == 12; }
 
 template <typename R>
-int tile_lines_rt(int n)
+int tile_lines_rt(int n, int tq)
 {
-#define PFDD_CASE(n_) if (n == n_) return tile_lines<R, n_>();
+#define PFDD_CASE(n_)                                              \
+    if (n == n_) {                                                 \
+        if (tq == 0) return tile_lines_q<R, n_, 0>();              \
+        if (tq == 2) return tile_lines_q<R, n_, 2>();              \
+        return tile_lines_q<R, n_, 1>();                           \
+    }
     PFDD_SIZES(PFDD_CASE)
 #undef PFDD_CASE
     return 0;
 }
 
+// Dispatch of a tile-shape choice tq (0, 1, 2) to the template argument.
+#define PFDD_TQ_DISPATCH(tq, ...)                                  \
+    switch (tq) {                                                  \
+    case 0: { constexpr int TQ = 0; __VA_ARGS__; } break;          \
+    case 2: { constexpr int TQ = 2; __VA_ARGS__; } break;          \
+    default: { constexpr int TQ = 1; __VA_ARGS__; } break;         \
+    }
+
 template <typename R, bool INV>
 hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo, long long nlines,
-                       const void *tw, const int *done, hipStream_t st, const uint8_t *zmask = nullptr)
+                       const void *tw, const int *done, hipStream_t st, const uint8_t *zmask = nullptr,
+                       int tq = 1)
 {
     typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                           \
-    if (n == n_) {                                                                              \
-        constexpr int NL = tile_lines<R, n_>();                                                 \
-        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
-        hipLaunchKernelGGL((k_fft_pass<R, n_, INV>), dim3(nb), dim3(NTHR), 0, st, (const V *)in, \
-                           (V *)out, mi, mo, nlines, (const V *)tw, done, zmask);               \
-        return hipGetLastError();                                                               \
+#define PFDD_CASE(n_)                                                                               \
+    if (n == n_) {                                                                                  \
+        PFDD_TQ_DISPATCH(tq, {                                                                      \
+            constexpr int NL = tile_lines_q<R, n_, TQ>();                                           \
+            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
+            hipLaunchKernelGGL((k_fft_pass<R, n_, INV, TQ>), dim3(nb), dim3(NTHR), 0, st,           \
+                               (const V *)in, (V *)out, mi, mo, nlines, (const V *)tw, done, zmask); \
+        })                                                                                          \
+        return hipGetLastError();                                                                   \
     }
     PFDD_SIZES(PFDD_CASE)
 #undef PFDD_CASE
     return hipErrorInvalidValue;
 }
 
-template <typename R, typename RA>
-hipError_t launch_update(int n, const void *T1, const void *xo, void *xn, const uint8_t *mask,
+template <typename R>
+hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn, const uint8_t *mask,
                          const long long *prev, const void *amp, LineMap mi, LineMap mo, long long nlines,
                          long long lps, const StepParams &P, const void *tw, double *partial,
                          long long pstride, const int *done, hipStream_t st)
 {
     typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                            \
-    if (n == n_) {                                                                               \
-        constexpr int NL = tile_lines<R, n_>();                                                  \
-        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
-        hipLaunchKernelGGL((k_update<R, RA, n_>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,    \
-                           (const V *)xo, (V *)xn, mask, prev, (const RA *)amp, mi, mo, nlines, lps, \
-                           P, (const V *)tw, partial, pstride, done);                            \
-        return hipGetLastError();                                                                \
+#define PFDD_CASE(n_)                                                                                \
+    if (n == n_) {                                                                                   \
+        PFDD_TQ_DISPATCH(tq, {                                                                       \
+            constexpr int NL = tile_lines_q<R, n_, TQ>();                                            \
+            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
+            hipLaunchKernelGGL((k_update<R, n_, TQ>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,    \
+                               (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, \
+                               P, (const V *)tw, partial, pstride, done);                            \
+        })                                                                                           \
+        return hipGetLastError();                                                                    \
     }
     PFDD_SIZES(PFDD_CASE)
 #undef PFDD_CASE
     return hipErrorInvalidValue;
 }
 
-template <typename R, typename RA>
-hipError_t launch_adj_update(int n, const void *T1, const void *lam, const void *xs, void *lam_out,
+template <typename R>
+hipError_t launch_adj_update(int n, int tq, const void *T1, const void *lam, const void *xs, void *lam_out,
                              const uint8_t *mask, const void *amp, LineMap mi, LineMap mo, long long nlines,
                              long long lps, const StepParams &P, const void *tw, double *gpart,
                              long long gstride, const int *done, hipStream_t st)
 {
     typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                               \
-    if (n == n_) {                                                                                  \
-        constexpr int NL = tile_lines<R, n_>();                                                     \
-        const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                     \
-        hipLaunchKernelGGL((k_adj_update<R, RA, n_>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,   \
-                           (const V *)lam, (const V *)xs, (V *)lam_out, mask, (const RA *)amp, mi, mo, \
-                           nlines, lps, P, (const V *)tw, gpart, gstride, done);                    \
-        return hipGetLastError();                                                                   \
+#define PFDD_CASE(n_)                                                                                  \
+    if (n == n_) {                                                                                     \
+        PFDD_TQ_DISPATCH(tq, {                                                                         \
+            constexpr int NL = tile_lines_q<R, n_, TQ>();                                              \
+            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                    \
+            hipLaunchKernelGGL((k_adj_update<R, n_, TQ>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,  \
+                               (const V *)lam, (const V *)xs, (V *)lam_out, mask, (const R *)amp, mi, mo, \
+                               nlines, lps, P, (const V *)tw, gpart, gstride, done);                   \
+        })                                                                                             \
+        return hipGetLastError();                                                                      \
     }
     PFDD_SIZES(PFDD_CASE)
 #undef PFDD_CASE
@@ -1150,7 +1174,8 @@ struct pfdd_gl {
     void *amp = nullptr;
     double *partial = nullptr;
     long long pstride = 0;
-    int nb3 = 0, nb5 = 0;
+    int nb3 = 0, nb5 = 0, nb5max = 0; // blocks of pass 3 / pass 5 (current tile, smallest tile)
+    int tq[4] = {1, 1, 1, 1};          // tile shape of fwd z, fwd y, inv y, inv z + update
     double *stage2 = nullptr; // NSLOT * NB2
     unsigned *ticket = nullptr;
     bool stored_B = false;    // stream packed B(k) from HBM instead of evaluating it on the fly
@@ -1175,7 +1200,7 @@ struct pfdd_gl {
         std::vector<void *> ckpt;   // xi at steps 0, every, 2*every, ...
         std::vector<void *> seg;    // recomputed states of one segment (every - 1)
         void *lam[2] = {nullptr, nullptr};
-        double *gpart = nullptr;    // (1 + MAXM) * nb5 per-block sums
+        double *gpart = nullptr;    // (1 + MAXM) * nb5max per-block sums
         double *gacc = nullptr;     // S * (1 + MAXM) accumulated sums
     } tape;
 };
@@ -1259,9 +1284,11 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     const Maps m = make_maps(h, h->SM);
     hipStream_t st = h->stream;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st)));
+    HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, nullptr,
+                                  h->tq[0])));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st, nullptr,
+                                  h->tq[1])));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     if (h->stored_B)
         HIPCHK((launch_mid<R>(h->nx, h->S, h->M, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
@@ -1270,18 +1297,11 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
         HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
                                   h->partial + 4 * h->pstride, h->done, st)));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
+                                 h->tq[2])));
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
-    hipError_t e;
-    if (h->prec == PFDD_FP64)
-        e = launch_update<R, double>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
-                                     (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride,
-                                     h->done, st);
-    else
-        e = launch_update<R, float>(h->nz, h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
-                                    (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride,
-                                    h->done, st);
-    HIPCHK(e);
+    HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
+                            (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done, st));
     if (ev) HIPCHK(hipEventRecord(ev[5], st));
     hipLaunchKernelGGL(k_reduce, dim3(NB2), dim3(NTHR), 0, st, h->partial, h->pstride, h->nb5, h->nb3,
                        1.0 / (double)h->N, 1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done,
@@ -1301,6 +1321,116 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
     return PFDD_OK;
 }
 
+// ---------------------------------------------------------------- autotune
+void set_tiles(pfdd_gl *h, const int tq[4])
+{
+    for (int p = 0; p < 4; ++p) h->tq[p] = tq[p];
+    const long long n5 = (long long)h->SM * h->nx * h->ny;
+    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, tq[3]) : tile_lines_rt<float>(h->nz, tq[3]);
+    h->nb5 = (int)((n5 + nl - 1) / nl);
+    ++h->version; // captured graphs hold the old launch shapes
+}
+
+// Per-pass device time (ms) of the four tunable passes over nsteps GL
+// iterations with the handle's current tile choices.
+template <typename R>
+int time_passes(pfdd_gl *h, int nsteps, double acc[4])
+{
+    hipEvent_t ev[7];
+    for (int q = 0; q < 7; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    for (int p = 0; p < 4; ++p) acc[p] = 0.0;
+    const int pass_ev[4] = {0, 1, 3, 4}; // fwd z, fwd y, inv y, inv z + update
+    int rc = PFDD_OK;
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        rc = step_once<R>(h, 0.0, ev);
+        if (rc) break;
+        if (hipEventSynchronize(ev[6]) != hipSuccess) {
+            rc = fail(PFDD_ERR_HIP, "autotune timing failed");
+            break;
+        }
+        for (int p = 0; p < 4; ++p) {
+            float t = 0;
+            hipEventElapsedTime(&t, ev[pass_ev[p]], ev[pass_ev[p] + 1]);
+            acc[p] += t;
+        }
+    }
+    for (int q = 0; q < 7; ++q) (void)hipEventDestroy(ev[q]);
+    return rc;
+}
+
+struct TuneKey {
+    int nx, ny, nz, S, M, prec, flags;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(nx, ny, nz, S, M, prec, flags) < std::tie(o.nx, o.ny, o.nz, o.S, o.M, o.prec, o.flags);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 4>> g_tune_cache;
+
+// Times every tile shape (0, 1, 2) of each line pass and keeps the fastest per
+// pass.  The field, ping-pong parity and step counters are restored.  Choices
+// that would break the reverse mode's one-system-per-block rule for the
+// update pass are skipped.
+template <typename R>
+int autotune(pfdd_gl *h, int nsteps, int choice[4])
+{
+    const TuneKey key{h->nx, h->ny, h->nz, h->S, h->M, h->prec,
+                      (h->stored_B ? 1 : 0) | (h->mask ? 2 : 0) | (h->amp ? 4 : 0)};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            set_tiles(h, it->second.data());
+            if (choice)
+                for (int p = 0; p < 4; ++p) choice[p] = it->second[p];
+            return PFDD_OK;
+        }
+    }
+    const size_t fb = (size_t)h->SM * h->N * h->esz;
+    void *save = nullptr;
+    HIPCHK(hipMalloc(&save, fb));
+    const int cur0 = h->cur;
+    const bool stepped0 = h->stepped;
+    HIPCHK(hipMemcpyAsync(save, h->xi[cur0], fb, hipMemcpyDeviceToDevice, h->stream));
+    double best[4] = {1e300, 1e300, 1e300, 1e300};
+    int arg[4] = {1, 1, 1, 1};
+    int rc = PFDD_OK;
+    for (int q = 0; q < 3 && !rc; ++q) {
+        const int all[4] = {q, q, q, q};
+        set_tiles(h, all);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+        double warm[4], t[4];
+        if ((rc = time_passes<R>(h, 1, warm)) || (rc = time_passes<R>(h, nsteps, t))) break;
+        const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, q) : tile_lines_rt<float>(h->nz, q);
+        const bool upd_ok = ((long long)h->nx * h->ny) % nl == 0;
+        for (int p = 0; p < 4; ++p)
+            if (t[p] < best[p] && (p < 3 || upd_ok)) {
+                best[p] = t[p];
+                arg[p] = q;
+            }
+    }
+    HIPCHK(hipMemcpyAsync(h->xi[cur0], save, fb, hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(save));
+    h->cur = cur0;
+    h->stepped = stepped0;
+    if (rc) {
+        const int dflt[4] = {1, 1, 1, 1};
+        set_tiles(h, dflt);
+        return rc;
+    }
+    set_tiles(h, arg);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = {arg[0], arg[1], arg[2], arg[3]};
+    }
+    if (choice)
+        for (int p = 0; p < 4; ++p) choice[p] = arg[p];
+    return PFDD_OK;
+}
+
 // One reverse-mode iteration: lam_in = dL/dxi_{n+1} -> lam_out = dL/dxi_n,
 // with xs = xi_n, accumulating dL/dtau and dL/dcoef sums into tape.gacc.
 template <typename R>
@@ -1308,26 +1438,22 @@ int adjoint_core(pfdd_gl *h, const void *xs, const void *lam_in, void *lam_out, 
 {
     const Maps m = make_maps(h, h->SM);
     hipStream_t st = h->stream;
-    HIPCHK((launch_pass<R, false>(h->nz, lam_in, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, h->mask)));
-    HIPCHK((launch_pass<R, false>(h->ny, h->T1, lam_out, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<R, false>(h->nz, lam_in, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, h->mask,
+                                  h->tq[0])));
+    HIPCHK((launch_pass<R, false>(h->ny, h->T1, lam_out, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st, nullptr,
+                                  h->tq[1])));
     if (h->stored_B)
         HIPCHK((launch_mid<R>(h->nx, h->S, h->M, lam_out, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
                               h->partial + 4 * h->pstride, h->done, st)));
     else
         HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, lam_out, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
                                   h->partial + 4 * h->pstride, h->done, st)));
-    HIPCHK((launch_pass<R, true>(h->ny, lam_out, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st)));
-    const long long gstride = h->nb5;
-    hipError_t e;
-    if (h->prec == PFDD_FP64)
-        e = launch_adj_update<R, double>(h->nz, h->T1, lam_in, xs, lam_out, h->mask, h->amp, m.p5i, m.p5o, m.n5,
-                                         (long long)h->nx * h->ny, P, h->tw[2], h->tape.gpart, gstride,
-                                         h->done, st);
-    else
-        e = launch_adj_update<R, float>(h->nz, h->T1, lam_in, xs, lam_out, h->mask, h->amp, m.p5i, m.p5o, m.n5,
-                                        (long long)h->nx * h->ny, P, h->tw[2], h->tape.gpart, gstride,
-                                        h->done, st);
-    HIPCHK(e);
+    HIPCHK((launch_pass<R, true>(h->ny, lam_out, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
+                                 h->tq[2])));
+    const long long gstride = h->nb5max;
+    HIPCHK(launch_adj_update<R>(h->nz, h->tq[3], h->T1, lam_in, xs, lam_out, h->mask, h->amp, m.p5i, m.p5o,
+                                m.n5, (long long)h->nx * h->ny, P, h->tw[2], h->tape.gpart, gstride, h->done,
+                                st));
     hipLaunchKernelGGL(k_grad_reduce, dim3(h->S), dim3(NTHR), 0, st, h->tape.gpart, gstride, h->nb5 / h->S,
                        h->tape.gacc, h->done);
     HIPCHK(hipGetLastError());
@@ -1617,7 +1743,7 @@ int tape_backward_any(pfdd_gl *h, const void *dxi_T, bool dev_in, void *dxi_0, b
         T.seg.push_back(p);
     }
     constexpr int NG = 1 + MAXM;
-    if (!T.gpart) HIPCHK(hipMalloc((void **)&T.gpart, sizeof(double) * NG * h->nb5));
+    if (!T.gpart) HIPCHK(hipMalloc((void **)&T.gpart, sizeof(double) * NG * h->nb5max));
     if (!T.gacc) HIPCHK(hipMalloc((void **)&T.gacc, sizeof(double) * NG * MAXS));
     const long long n = (long long)h->S * h->N;
     // seed
@@ -1779,8 +1905,10 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
 
     // per-block partials
     const long long n5 = (long long)h->SM * h->nx * h->ny;
-    const int nl5 = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
+    const int nl5 = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, 1) : tile_lines_rt<float>(h->nz, 1);
+    const int nl5min = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, 0) : tile_lines_rt<float>(h->nz, 0);
     h->nb5 = (int)((n5 + nl5 - 1) / nl5);
+    h->nb5max = (int)((n5 + nl5min - 1) / nl5min);
     h->nb3 = h->M * (h->prec == PFDD_FP64 ? mid_blocks<double>(h->nx, h->S, (long long)h->ny * h->nz)
                                           : mid_blocks<float>(h->nx, h->S, (long long)h->ny * h->nz));
     if (h->M > 1) {
@@ -1789,7 +1917,7 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
         if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
         h->P.taud = h->taud;
     }
-    h->pstride = h->nb5 > h->nb3 ? h->nb5 : h->nb3;
+    h->pstride = h->nb5max > h->nb3 ? h->nb5max : h->nb3;
     if ((rc = alloc((void **)&h->partial, sizeof(double) * NSLOT * h->pstride))) return bail(rc);
     if ((rc = alloc((void **)&h->res, sizeof(double) * NRES))) return bail(rc);
     if ((rc = alloc((void **)&h->stage2, sizeof(double) * NSLOT * NB2)) ||
@@ -1840,8 +1968,20 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     if (rc) return bail(rc);
     e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "setup: %s", hipGetErrorString(e)));
+    if (d->flags & PFDD_GL_FLAG_AUTOTUNE) {
+        rc = h->prec == PFDD_FP64 ? autotune<double>(h, 4, nullptr) : autotune<float>(h, 4, nullptr);
+        if (rc) return bail(rc);
+    }
     *out = h;
     return PFDD_OK;
+}
+
+int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[4])
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (nsteps < 1) return fail(PFDD_ERR_ARG, "nsteps must be >= 1");
+    return h->prec == PFDD_FP64 ? autotune<double>(h, nsteps, choice) : autotune<float>(h, nsteps, choice);
 }
 
 int pfdd_gl_destroy(pfdd_gl *h)
@@ -1944,7 +2084,7 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
     if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
-    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz) : tile_lines_rt<float>(h->nz);
+    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3]) : tile_lines_rt<float>(h->nz, h->tq[3]);
     if (((long long)h->nx * h->ny) % nl != 0)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);
     pfdd_gl::Tape &T = h->tape;

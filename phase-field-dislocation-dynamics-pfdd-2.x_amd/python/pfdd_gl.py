@@ -20,7 +20,9 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfdd_gl.so")
 PFDD_FP64, PFDD_FP32 = 0, 1
 CORE_NONE, CORE_PERFECT, CORE_SINE, CORE_MPEA, CORE_SERIES = 0, 1, 3, 7, 8
 MAX_SYSTEMS = 12
+MAX_COEF = 8
 FLAG_STORED_B = 1
+FLAG_AUTOTUNE = 2
 STATUS = {0: "PFDD_OK", 1: "PFDD_ERR_ARG", 2: "PFDD_ERR_UNSUPPORTED", 3: "PFDD_ERR_HIP",
           4: "PFDD_ERR_NONFINITE", 5: "PFDD_ERR_STATE"}
 
@@ -71,6 +73,7 @@ EXPORTS = [
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
     "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
+    "pfdd_gl_autotune",
 ]
 
 _lib = None
@@ -112,6 +115,7 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_graph_create": (c_int, [ctypes.POINTER(vp), c_int, c_int, ctypes.POINTER(vp)]),
         "pfdd_gl_graph_run": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
         "pfdd_gl_graph_destroy": (c_int, [vp]),
+        "pfdd_gl_autotune": (c_int, [vp, c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -139,7 +143,7 @@ class GLSolver:
 
     def __init__(self, shape, xn, xb, four, C44, C12, C11, dslip, cddt, *, S12=0.0, S44=0.0,
                  core_model=CORE_NONE, coef=None, ncoef=0, amp=None, xo=None,
-                 precision=PFDD_FP64, device=0, stored_B=False, batch=1):
+                 precision=PFDD_FP64, device=0, stored_B=False, batch=1, autotune=False):
         L = load_library()
         self.shape = tuple(int(v) for v in shape)
         self._xn, self._xb = _f64(xn).reshape(-1, 3), _f64(xb).reshape(-1, 3)
@@ -163,7 +167,7 @@ class GLSolver:
         d.coef, d.amp, d.xo = _ptr(self._coef), _ptr(self._amp), _ptr(self._xo)
         d.device = device
         d.stream = None
-        d.flags = FLAG_STORED_B if stored_B else 0
+        d.flags = (FLAG_STORED_B if stored_B else 0) | (FLAG_AUTOTUNE if autotune else 0)
         h = ctypes.c_void_p()
         _check(L.pfdd_gl_create_batch(ctypes.byref(d), self.batch, ctypes.byref(h)))
         self._h = h
@@ -224,6 +228,13 @@ class GLSolver:
                                       ctypes.byref(r) if sync else None))
         return r.as_dict() if sync else None
 
+    def autotune(self, nsteps=4):
+        """Pick the fastest LDS tile shape per line pass; returns the choices
+        (fwd z, fwd y, inv y, inv z + update), each 0, 1 or 2."""
+        c = (ctypes.c_int * 4)()
+        _check(self._lib.pfdd_gl_autotune(self._h, int(nsteps), c))
+        return list(c)
+
     # --------------------------------------------------------- reverse mode
     def tape_record(self, nsteps, every):
         """nsteps GL iterations (as step(nsteps)) keeping checkpoints every `every`."""
@@ -237,7 +248,7 @@ class GLSolver:
         assert g.shape == (self.S,) + self.shape, g.shape
         dxi0 = np.empty_like(g)
         dtau = np.zeros(self.S)
-        dcoef = np.zeros(self.S * max(int(ncoef), 1))
+        dcoef = np.zeros(self.S * MAX_COEF)  # the library writes S * (its ncoef) <= S * MAX_COEF
         _check(self._lib.pfdd_gl_tape_backward(self._h, g.ctypes.data_as(_dp), dxi0.ctypes.data_as(_dp),
                                                _ptr(dtau), _ptr(dcoef)))
         return dxi0, dtau, dcoef[: self.S * int(ncoef)].reshape(self.S, int(ncoef))
@@ -245,7 +256,7 @@ class GLSolver:
     def tape_backward_device(self, dxi_T_ptr: int, dxi_0_ptr: int, ncoef):
         """As tape_backward with device buffers (complex128 layout)."""
         dtau = np.zeros(self.S)
-        dcoef = np.zeros(self.S * max(int(ncoef), 1))
+        dcoef = np.zeros(self.S * MAX_COEF)
         _check(self._lib.pfdd_gl_tape_backward_device(self._h, ctypes.c_void_p(dxi_T_ptr),
                                                       ctypes.c_void_p(dxi_0_ptr), _ptr(dtau), _ptr(dcoef)))
         return dtau, dcoef[: self.S * int(ncoef)].reshape(self.S, int(ncoef))

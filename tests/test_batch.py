@@ -118,3 +118,28 @@ def test_batch_rejects_single_run_only_calls():
     with pytest.raises(pfdd_gl.PfddError, match="UNSUPPORTED"):
         b.tape_record(2, 1)
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_results(precision):
+    print(f"case: autotune 32^3 x 3 precision {precision}: fields equal to the default tiles, "
+          "state restored", flush=True)
+    c = member_cases((32, 32, 32), 1)[0]
+    s0 = P.gpu_solver(c, precision=precision)
+    s1 = P.gpu_solver(c, precision=precision)
+    s1.step(3)
+    x_before = s1.get_xi()
+    choice = s1.autotune(3)
+    assert len(choice) == 4 and all(q in (0, 1, 2) for q in choice)
+    assert np.array_equal(s1.get_xi(), x_before)  # tuning restores the field
+    s0.step(3)
+    s0.step(17)
+    s1.step(17)
+    assert np.array_equal(s0.get_xi(), s1.get_xi())
+    # the reverse mode runs with the tuned tiles
+    s1.tape_record(4, 2)
+    lam, dtau, _ = s1.tape_backward(np.ones((3, 32, 32, 32), dtype=np.complex128), 0)
+    assert np.all(np.isfinite(lam)) and np.all(np.isfinite(dtau))
+    s0.close()
+    s1.close()
