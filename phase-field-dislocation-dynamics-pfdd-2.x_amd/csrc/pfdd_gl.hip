@@ -319,19 +319,25 @@ template <typename R, int N, int S> constexpr int otf_tpp()
     return (2 * pts == NTHR && S % 2 == 0) ? 2 : 1;
 }
 
-// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im).
-template <int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
-__device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const double f[3], double f2,
-                                          const double *__restrict__ cst, const GParams &G, Sink sink)
+// Scalars of the contraction in the arithmetic type T of the x pass.
+template <typename T> struct GConst {
+    T inv2C44, inv1mnu, invmu, C44x2, C12;
+};
+
+// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im),
+// computed in T (double in the FP64 path, float in the FP32 mode).
+template <typename T, int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
+__device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const T f[3], T f2,
+                                          const T *__restrict__ cst, const GConst<T> &G, Sink sink)
 {
     // compiler-only barriers keep the (LDS-broadcast) constants from being
     // hoisted and held in registers across points and systems
     __asm__ volatile("" ::: "memory");
-    double Pr[6] = {0, 0, 0, 0, 0, 0}, Pi[6] = {0, 0, 0, 0, 0, 0}; // Sig, then P
+    T Pr[6] = {}, Pi[6] = {}; // Sig, then P
 #pragma unroll
     for (int b = 0; b < S; ++b) {
         const V v = lds.ld(p * LDP + b * NP + q);
-        const double xr = (double)v.x, xi = (double)v.y;
+        const T xr = (T)v.x, xi = (T)v.y;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             Pr[c] += cst[b * 12 + c] * xr;
@@ -340,36 +346,36 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
     }
     if (f2 > 0) {
         // symmetric components xx, yy, zz, xy, xz, yz
-        const double wr[3] = {Pr[0] * f[0] + Pr[3] * f[1] + Pr[4] * f[2], Pr[3] * f[0] + Pr[1] * f[1] + Pr[5] * f[2],
+        const T wr[3] = {Pr[0] * f[0] + Pr[3] * f[1] + Pr[4] * f[2], Pr[3] * f[0] + Pr[1] * f[1] + Pr[5] * f[2],
                               Pr[4] * f[0] + Pr[5] * f[1] + Pr[2] * f[2]};
-        const double wi[3] = {Pi[0] * f[0] + Pi[3] * f[1] + Pi[4] * f[2], Pi[3] * f[0] + Pi[1] * f[1] + Pi[5] * f[2],
+        const T wi[3] = {Pi[0] * f[0] + Pi[3] * f[1] + Pi[4] * f[2], Pi[3] * f[0] + Pi[1] * f[1] + Pi[5] * f[2],
                               Pi[4] * f[0] + Pi[5] * f[1] + Pi[2] * f[2]};
-        const double Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
-        const double Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
-        const double if2 = 1.0 / f2;
-        const double c2 = if2 * G.inv2C44;                 // multiplies (w_i f_j + w_j f_i)
-        const double c4 = G.inv1mnu * if2 * if2 * G.inv2C44; // multiplies V f_i f_j
+        const T Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
+        const T Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
+        const T if2 = (T)1 / f2;
+        const T c2 = if2 * G.inv2C44;                 // multiplies (w_i f_j + w_j f_i)
+        const T c4 = G.inv1mnu * if2 * if2 * G.inv2C44; // multiplies V f_i f_j
         const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
-        double Qr[6], Qi[6];
+        T Qr[6], Qi[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            const double ff = f[ci[c]] * f[cj[c]];
+            const T ff = f[ci[c]] * f[cj[c]];
             Qr[c] = c2 * (wr[ci[c]] * f[cj[c]] + wr[cj[c]] * f[ci[c]]) - c4 * Vr * ff;
             Qi[c] = c2 * (wi[ci[c]] * f[cj[c]] + wi[cj[c]] * f[ci[c]]) - c4 * Vi * ff;
         }
-        const double tr_r = G.C12 * (Qr[0] + Qr[1] + Qr[2]), tr_i = G.C12 * (Qi[0] + Qi[1] + Qi[2]);
+        const T tr_r = G.C12 * (Qr[0] + Qr[1] + Qr[2]), tr_i = G.C12 * (Qi[0] + Qi[1] + Qi[2]);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            Pr[c] -= G.C44x2 * Qr[c] + (c < 3 ? tr_r : 0.0);
-            Pi[c] -= G.C44x2 * Qi[c] + (c < 3 ? tr_i : 0.0);
+            Pr[c] -= G.C44x2 * Qr[c] + (c < 3 ? tr_r : (T)0);
+            Pi[c] -= G.C44x2 * Qi[c] + (c < 3 ? tr_i : (T)0);
         }
     }
 #pragma unroll
     for (int aa = 0; aa < SA; ++aa) {
         __asm__ volatile("" ::: "memory");
         const int a = H * SA + aa;
-        const double *ep = cst + a * 12 + 6;
-        double gr = 0.0, gi = 0.0;
+        const T *ep = cst + a * 12 + 6;
+        T gr = 0, gi = 0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             gr += Pr[c] * ep[c];
@@ -377,23 +383,36 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
         }
         gr *= G.invmu;
         gi *= G.invmu;
-        if (!(f2 > 0)) gr = gi = 0.0;
+        if (!(f2 > 0)) gr = gi = (T)0;
         sink(aa, gr, gi);
     }
 }
 
-template <int N, int NP, int S>
-__device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p, double f[3], double &f2)
+// f = four^T k~ of k-point (p, col) (frec, fft_fftw_slab.cpp:280-367), formed
+// in FP64 and rounded once to T.
+template <int N, int NP, int S, typename T>
+__device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p, T f[3], T &f2)
 {
     const int k = (int)(col / G.ny), j = (int)(col % G.ny);
     const double kx = frac_freq(p, G.nx), ky = frac_freq(j, G.ny), kz = frac_freq(k, G.nz);
+    double fd[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[c] = kx * G.four[c] + ky * G.four[3 + c] + kz * G.four[6 + c];
+    for (int c = 0; c < 3; ++c) fd[c] = kx * G.four[c] + ky * G.four[3 + c] + kz * G.four[6 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = (T)fd[c];
     f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
 }
 
+// Minimum waves per SIMD asked of the x pass: 5 for the 12-system FP64 cases
+// (their LDS tile allows 5 blocks per CU, and the pass is latency-bound; the
+// register budget of 96 then holds without spills), no constraint otherwise.
+template <typename R, int N, int S> constexpr int otf_waves()
+{
+    return (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) ? 5 : 1;
+}
+
 template <typename R, int N, int S>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_waves<R, N, S>(), 8)))
     k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
               const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
               const int *__restrict__ done)
@@ -405,16 +424,18 @@ __global__ void __launch_bounds__(NTHR)
     constexpr int TPP = otf_tpp<R, N, S>();
     constexpr int SA = S / TPP;
     constexpr int NPT = NP * N; // k-points per block
+    typedef R T; // arithmetic type of the contraction
     PFDD_TILE(V, lds, N * LDP);
-    __shared__ double cst[S * 12];
+    __shared__ T cst[S * 12];
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
     // member of a batch (bpm blocks per member), first column of the block
     const int mem = blockIdx.x / bpm;
     const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
     T2 += (long long)mem * S * ntot;
-    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
-                                                                    : G.eps6[(c / 12) * 6 + c % 12 - 6];
+    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (T)((c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
+                                                                       : G.eps6[(c / 12) * 6 + c % 12 - 6]);
+    const GConst<T> gc{(T)G.inv2C44, (T)G.inv1mnu, (T)G.invmu, (T)G.C44x2, (T)G.C12};
     tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
@@ -427,9 +448,9 @@ __global__ void __launch_bounds__(NTHR)
         for (int pt = tid; pt < NPT; pt += NTHR) {
             const int q = pt / N, p = pt % N;
             if (c0 + q >= ncol) continue;
-            double f[3], f2;
+            T f[3], f2;
             otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
-            otf_point<S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, G, [&](int a, double gr, double gi) {
+            otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, gc, [&](int a, T gr, T gi) {
                 const int idx = p * LDP + a * NP + q;
                 const V x = lds.ld(idx);
                 el += (double)x.x * gr + (double)x.y * gi;
@@ -442,16 +463,18 @@ __global__ void __launch_bounds__(NTHR)
         const int h = tid / NPT, pt = tid % NPT;
         const int q = pt / N, p = pt % N;
         const bool ok = c0 + q < ncol;
-        double f[3], f2;
+        T f[3], f2;
         otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
-        double g[SA][2];
-        auto keep = [&](int aa, double gr, double gi) {
+        T g[SA][2];
+        auto keep = [&](int aa, T gr, T gi) {
             g[aa][0] = gr;
             g[aa][1] = gi;
         };
         if (ok) {
-            if (__builtin_amdgcn_readfirstlane(h) == 0) otf_point<S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, G, keep);
-            else otf_point<S, SA, 1, NP, LDP>(lds, p, q, f, f2, cst, G, keep);
+            if (__builtin_amdgcn_readfirstlane(h) == 0)
+                otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, gc, keep);
+            else
+                otf_point<T, S, SA, 1, NP, LDP>(lds, p, q, f, f2, cst, gc, keep);
         }
         __syncthreads();
         if (ok) {
@@ -497,7 +520,9 @@ __global__ void __launch_bounds__(NTHR)
     constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
-    __shared__ int lsys[NL], lcs[NL]; // system of the line in the batch / within its member
+    // per-line tau and core coefficients: read once per block here, since
+    // kernel-argument arrays indexed per lane compile to per-lane memory loads
+    __shared__ R ltau[NL], lcoef[MAXM * NL];
     __shared__ double red[4 * (NTHR / 64)];
     const int tid = threadIdx.x;
     const long long L0 = (long long)blockIdx.x * NL;
@@ -506,8 +531,10 @@ __global__ void __launch_bounds__(NTHR)
         const bool ok = L < nlines;
         bin[l] = ok ? line_base(mi, L) : -1;
         bout[l] = ok ? line_base(mo, L) : -1;
-        lsys[l] = ok ? (int)(L / lines_per_sys) : 0;
-        lcs[l] = lsys[l] % P.S;
+        const int a = ok ? (int)(L / lines_per_sys) : 0; // system in the batch
+        const int as = a % P.S;                           // system within its member
+        ltau[l] = (R)(P.taud ? P.taud[a] : P.tau[a]);
+        for (int m = 0; m < MAXM; ++m) lcoef[m * NL + l] = (R)P.coef[as * MAXM + m];
     }
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
@@ -547,7 +574,6 @@ __global__ void __launch_bounds__(NTHR)
         const long long b = bout[l];
         if (b < 0) continue;
         const long long idx = b + p;
-        const int a = lsys[l], as = lcs[l];
         const V g = lds.ld(p * LDP + l);
         V x;
         x = xold[t];
@@ -557,7 +583,7 @@ __global__ void __launch_bounds__(NTHR)
         else sincospif((float)x.x, (float *)&sp, (float *)&cp);
         R sm = sp, cm = cp, dens = (R)0, dE = (R)0;
         for (int m = 1; m <= nm; ++m) {
-            const R c = (R)P.coef[as * MAXM + m - 1];
+            const R c = lcoef[(m - 1) * NL + l];
             dens += c * sm * sm;
             dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
             const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
@@ -568,7 +594,7 @@ __global__ void __launch_bounds__(NTHR)
         if (amp) A = (R)amp[idx];
         dens *= A;
         dE *= A;
-        const R tau = (R)(P.taud ? P.taud[a] : P.tau[a]);
+        const R tau = ltau[l];
         V y;
         double dn;
         if (mask && mask[idx]) {
