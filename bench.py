@@ -62,8 +62,12 @@ def time_steps(s, steps, warmup):
     return (t1 - t0), r
 
 
-def roofline(s, prof_steps=20):
-    prof = s.profile(prof_steps)  # ms per kernel, HIP events on the solver's stream
+def roofline(s, prof_steps=50, warm_steps=50):
+    # steady state first (the clocks settle within a few tens of iterations of
+    # continuous load), then per-kernel device times from HIP events that the
+    # kernel dispatches record on the solver's stream
+    s.step(warm_steps)
+    prof = s.profile(prof_steps)
     kbytes, step_bytes = s.traffic()
     names = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"]
     times = [prof[k] for k in names]
@@ -437,7 +441,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)

@@ -206,7 +206,8 @@ int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, dou
 int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[4]);
 
 /* Average device time (ms) of each kernel of one GL iteration over `nsteps`
- * iterations, measured with HIP events on the handle's stream.  ms[0..4] =
+ * iterations, measured with HIP events on the handle's stream that the kernel
+ * dispatches record themselves (start and end of each kernel).  ms[0..4] =
  * forward z pass, forward y pass, x pass with contraction, inverse y pass,
  * inverse z pass with update; ms[5] = reduction; ms[6] = whole iteration.
  * nkernels receives the number of kernels of one iteration. */

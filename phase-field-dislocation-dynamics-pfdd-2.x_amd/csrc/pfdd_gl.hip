@@ -24,6 +24,7 @@
 // forward_mode1 :1953-1964, internal_energy :1670-1695, backward_mode1
 // :2012-2023, update_order_parameter :1746-1790, prepare_next_itr :1838-1862,
 // init_loop :1628-1666 (no longer needed), Bmatrix :466-638, frec :280-367.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -51,6 +52,19 @@ constexpr int NSLOT = 5; // xinorm^2, xiave, e_core, e_stress, e_el
 constexpr int NRES = 8;  // xinorm, xiave, e_el, e_core, e_stress, e_total, e_core_ref, nonfinite
 
 thread_local std::string g_err;
+
+// Kernel-timing events for the next launch of the GL iteration's kernels
+// (set by the profiler around one launch at a time): the dispatch itself
+// records them (hipExtLaunchKernelGGL), so they bracket exactly the kernel.
+thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+
+#define PFDD_LAUNCH(kern, grid, block, st, ...)                                                        \
+    do {                                                                                              \
+        if (t_ev_start)                                                                               \
+            hipExtLaunchKernelGGL(kern, grid, block, 0, st, t_ev_start, t_ev_stop, 0, __VA_ARGS__);    \
+        else                                                                                          \
+            hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__);                                \
+    } while (0)
 
 int fail(int code, const char *fmt, ...)
 {
@@ -998,7 +1012,7 @@ hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo,
         PFDD_TQ_DISPATCH(tq, {                                                                      \
             constexpr int NL = tile_lines_q<R, n_, TQ>();                                           \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
-            hipLaunchKernelGGL((k_fft_pass<R, n_, INV, TQ>), dim3(nb), dim3(NTHR), 0, st,           \
+            PFDD_LAUNCH((k_fft_pass<R, n_, INV, TQ>), dim3(nb), dim3(NTHR), st,                     \
                                (const V *)in, (V *)out, mi, mo, nlines, (const V *)tw, done, zmask); \
         })                                                                                          \
         return hipGetLastError();                                                                   \
@@ -1020,7 +1034,7 @@ hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn
         PFDD_TQ_DISPATCH(tq, {                                                                       \
             constexpr int NL = tile_lines_q<R, n_, TQ>();                                            \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
-            hipLaunchKernelGGL((k_update<R, n_, TQ>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,    \
+            PFDD_LAUNCH((k_update<R, n_, TQ>), dim3(nb), dim3(NTHR), st, (const V *)T1,              \
                                (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, \
                                P, (const V *)tw, partial, pstride, done);                            \
         })                                                                                           \
@@ -1092,7 +1106,7 @@ hipError_t launch_mid_s(int n, int M, void *T2, const void *B, long long ncol, l
         if constexpr (mid_fits<R, n_, S>()) {                                                     \
             constexpr int NP = mid_pairs<R, n_, S>();                                             \
             const int bpm = (int)((ncol + NP - 1) / NP);                                          \
-            hipLaunchKernelGGL((k_mid<R, R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), 0, st, \
+            PFDD_LAUNCH((k_mid<R, R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,          \
                                (V *)T2, (const R *)B, ncol, ntot, bpm, (const V *)tw, part_el, done); \
             return hipGetLastError();                                                             \
         } else {                                                                                  \
@@ -1150,7 +1164,7 @@ hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long 
         if constexpr (mid_fits<R, n_, S>()) {                                                     \
             constexpr int NP = mid_pairs<R, n_, S>();                                             \
             const int bpm = (int)((ncol + NP - 1) / NP);                                          \
-            hipLaunchKernelGGL((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), 0, st, \
+            PFDD_LAUNCH((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,          \
                                (V *)T2, G, ncol, ntot, bpm, (const V *)tw, part_el, done);        \
             return hipGetLastError();                                                             \
         } else {                                                                                  \
@@ -1309,31 +1323,39 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
 {
     const Maps m = make_maps(h, h->SM);
     hipStream_t st = h->stream;
-    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    // ev (profiling): 2 events per kernel, [2k] at its start and [2k+1] at its end
+    auto arm = [&](int k) {
+        if (ev) {
+            t_ev_start = ev[2 * k];
+            t_ev_stop = ev[2 * k + 1];
+        }
+    };
+    struct Disarm {
+        ~Disarm() { t_ev_start = t_ev_stop = nullptr; }
+    } disarm;
+    arm(0);
     HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, nullptr,
                                   h->tq[0])));
-    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    arm(1);
     HIPCHK((launch_pass<R, false>(h->ny, h->T1, T2, m.p2i, m.p2o, m.n2, h->tw[1], h->done, st, nullptr,
                                   h->tq[1])));
-    if (ev) HIPCHK(hipEventRecord(ev[2], st));
+    arm(2);
     if (h->stored_B)
         HIPCHK((launch_mid<R>(h->nx, h->S, h->M, T2, h->B, (long long)h->ny * h->nz, h->N, h->tw[0],
                               h->partial + 4 * h->pstride, h->done, st)));
     else
         HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
                                   h->partial + 4 * h->pstride, h->done, st)));
-    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    arm(3);
     HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
                                  h->tq[2])));
-    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+    arm(4);
     HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
                             (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done, st));
-    if (ev) HIPCHK(hipEventRecord(ev[5], st));
-    hipLaunchKernelGGL(k_reduce, dim3(NB2), dim3(NTHR), 0, st, h->partial, h->pstride, h->nb5, h->nb3,
-                       1.0 / (double)h->N, 1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done,
-                       h->steps);
+    arm(5);
+    PFDD_LAUNCH(k_reduce, dim3(NB2), dim3(NTHR), st, h->partial, h->pstride, h->nb5, h->nb3, 1.0 / (double)h->N,
+                1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done, h->steps);
     HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[6], st));
     return PFDD_OK;
 }
 
@@ -1362,25 +1384,25 @@ void set_tiles(pfdd_gl *h, const int tq[4])
 template <typename R>
 int time_passes(pfdd_gl *h, int nsteps, double acc[4])
 {
-    hipEvent_t ev[7];
-    for (int q = 0; q < 7; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    hipEvent_t ev[12];
+    for (int q = 0; q < 12; ++q) HIPCHK(hipEventCreate(&ev[q]));
     for (int p = 0; p < 4; ++p) acc[p] = 0.0;
-    const int pass_ev[4] = {0, 1, 3, 4}; // fwd z, fwd y, inv y, inv z + update
+    const int pass_k[4] = {0, 1, 3, 4}; // fwd z, fwd y, inv y, inv z + update
     int rc = PFDD_OK;
     for (int s = 0; s < nsteps && !rc; ++s) {
         rc = step_once<R>(h, 0.0, ev);
         if (rc) break;
-        if (hipEventSynchronize(ev[6]) != hipSuccess) {
+        if (hipEventSynchronize(ev[11]) != hipSuccess) {
             rc = fail(PFDD_ERR_HIP, "autotune timing failed");
             break;
         }
         for (int p = 0; p < 4; ++p) {
             float t = 0;
-            hipEventElapsedTime(&t, ev[pass_ev[p]], ev[pass_ev[p] + 1]);
+            hipEventElapsedTime(&t, ev[2 * pass_k[p]], ev[2 * pass_k[p] + 1]);
             acc[p] += t;
         }
     }
-    for (int q = 0; q < 7; ++q) (void)hipEventDestroy(ev[q]);
+    for (int q = 0; q < 12; ++q) (void)hipEventDestroy(ev[q]);
     return rc;
 }
 
@@ -2250,27 +2272,28 @@ int pfdd_gl_profile(pfdd_gl *h, int nsteps, double *ms, int *nkernels)
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!ms || nsteps <= 0) return fail(PFDD_ERR_ARG, "need ms[7] and nsteps > 0");
-    hipEvent_t ev[7];
-    for (int q = 0; q < 7; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    // all iterations back to back (as in a production run), one wait at the end
+    std::vector<hipEvent_t> ev((size_t)12 * nsteps);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
-    for (int s = 0; s < nsteps; ++s) {
-        rc = h->prec == PFDD_FP64 ? step_once<double>(h, 0.0, ev) : step_once<float>(h, 0.0, ev);
-        if (rc) return rc;
-        HIPCHK(hipEventSynchronize(ev[6]));
+    for (int s = 0; s < nsteps && !rc; ++s)
+        rc = h->prec == PFDD_FP64 ? step_once<double>(h, 0.0, &ev[12 * s]) : step_once<float>(h, 0.0, &ev[12 * s]);
+    if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(PFDD_ERR_HIP, "profile wait failed");
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        const hipEvent_t *e = &ev[12 * s];
         for (int q = 0; q < 6; ++q) {
             float t = 0;
-            HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+            hipEventElapsedTime(&t, e[2 * q], e[2 * q + 1]);
             acc[q] += t;
         }
         float t = 0;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[6]));
+        hipEventElapsedTime(&t, e[0], e[11]);
         acc[6] += t;
     }
-    for (int q = 0; q < 7; ++q) {
-        ms[q] = acc[q] / nsteps;
-        hipEventDestroy(ev[q]);
-    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int q = 0; q < 7; ++q) ms[q] = acc[q] / nsteps;
     if (nkernels) *nkernels = 6;
     return PFDD_OK;
 }
