@@ -522,7 +522,7 @@ def main():
             "fp32": {"bytes": f32["roofline"]["step_bytes"],
                      "frac": f32["roofline"]["step_bytes"] / (f32["ms_per_step"] * 1e-3) / PEAK_HBM}},
         "per_kernel": {"fp64": f64["roofline"]["per_kernel"], "fp32": f32["roofline"]["per_kernel"]},
-        "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update"],
+        "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update", "x_contract (1 = pipelined)"],
                             "fp64": f64["roofline"]["tiles"], "fp32": f32["roofline"]["tiles"]},
     }
     print(json.dumps(line))

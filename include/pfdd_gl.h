@@ -199,11 +199,15 @@ int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, dou
 /* Autotuner (no counterpart in the reference): times each candidate LDS tile
  * shape (0, 1, 2: about 1K/2K/4K FP64 or 2K/4K/8K FP32 elements per tile) of
  * the four line passes -- forward z, forward y, inverse y, inverse z + update
- * -- over `nsteps` GL iterations on this handle's own buffers, and keeps the
- * fastest per pass (choice[4], may be NULL).  The field and step counters are
- * restored; the choice is cached for the process per grid, system count,
+ * -- and both x-pass variants (0: one tile per workgroup, 1: persistent
+ * workgroups that load the next tile while transforming the current one)
+ * over `nsteps` GL iterations on this handle's own buffers, and keeps the
+ * fastest per kernel (choice[5], may be NULL).  The field and step counters
+ * are restored; the choice is cached for the process per grid, system count,
  * batch, precision and options.  Results (xi) do not depend on the choice. */
-int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[4]);
+int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5]);
+/* Sets the choices explicitly (same meaning as pfdd_gl_autotune's output). */
+int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5]);
 
 /* Average device time (ms) of each kernel of one GL iteration over `nsteps`
  * iterations, measured with HIP events on the handle's stream that the kernel

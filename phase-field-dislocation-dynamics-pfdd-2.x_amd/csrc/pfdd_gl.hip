@@ -425,33 +425,21 @@ template <typename R, int N, int S> constexpr int otf_waves()
     return (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) ? 5 : 1;
 }
 
+// The transform of one x-pass tile held in LDS (NP columns x S systems):
+// forward FFT along x, on-the-fly contraction ghat_a = sum_b B_ab xihat_b,
+// inverse FFT.  Returns this thread's share of sum Re(conj(xihat_a) ghat_a).
 template <typename R, int N, int S>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_waves<R, N, S>(), 8)))
-    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
-              const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
-              const int *__restrict__ done)
+__device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> lds,
+                                           const typename cvec<R>::type *__restrict__ tw, const GParams &G,
+                                           const GConst<R> &gc, const R *cst, long long c0, long long ncol, int tid)
 {
     typedef typename cvec<R>::type V;
-    if (*done) return;
+    typedef R T;
     constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
+    constexpr int NL = S * NP, LDP = NL + 1;
     constexpr int TPP = otf_tpp<R, N, S>();
     constexpr int SA = S / TPP;
-    constexpr int NPT = NP * N; // k-points per block
-    typedef R T; // arithmetic type of the contraction
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ T cst[S * 12];
-    __shared__ double red[NTHR / 64];
-    const int tid = threadIdx.x;
-    // member of a batch (bpm blocks per member), first column of the block
-    const int mem = blockIdx.x / bpm;
-    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
-    T2 += (long long)mem * S * ntot;
-    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (T)((c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
-                                                                       : G.eps6[(c / 12) * 6 + c % 12 - 6]);
-    const GConst<T> gc{(T)G.inv2C44, (T)G.inv1mnu, (T)G.invmu, (T)G.C44x2, (T)G.C12};
-    tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
-    __syncthreads();
+    constexpr int NPT = NP * N; // k-points per tile
     fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
     // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
     // replaces xihat_a in the tile
@@ -503,15 +491,126 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+    return el;
+}
+
+template <typename R, int S>
+__device__ __forceinline__ void otf_consts(R *cst, GConst<R> &gc, const GParams &G, int tid)
+{
+    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (R)((c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
+                                                                       : G.eps6[(c / 12) * 6 + c % 12 - 6]);
+    gc = GConst<R>{(R)G.inv2C44, (R)G.inv1mnu, (R)G.invmu, (R)G.C44x2, (R)G.C12};
+}
+
+// x pass, one tile per block.
+template <typename R, int N, int S>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_waves<R, N, S>(), 8)))
+    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
+              const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
+              const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NP = mid_pairs<R, N, S>();
+    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
+    PFDD_TILE(V, lds, N * LDP);
+    __shared__ R cst[S * 12];
+    __shared__ double red[NTHR / 64];
+    const int tid = threadIdx.x;
+    // member of a batch (bpm blocks per member), first column of the block
+    const int mem = blockIdx.x / bpm;
+    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
+    T2 += (long long)mem * S * ntot;
+    GConst<R> gc;
+    otf_consts<R, S>(cst, gc, G, tid);
+    tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
+    __syncthreads();
+    double v[1] = {otf_tile<R, N, S>(lds, tw, G, gc, cst, c0, ncol, tid)};
 #pragma unroll 4
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
         if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
     }
-    double v[1] = {el};
     block_sum<1, NTHR>(v, red, tid);
     if (tid == 0) part_el[blockIdx.x] = v[0];
+}
+
+// x pass, persistent and software-pipelined: each block walks the tiles
+// t = blockIdx.x, + gridDim.x, ... and keeps the next tile's input in
+// registers, loaded while the current tile is transformed, so that memory
+// traffic and the FFT/contraction arithmetic of the resident blocks overlap
+// instead of running in lock-step phases.  Tile t is member t / bpm, columns
+// (t % bpm) * NP ...; part_el gets one partial per block.
+template <int N, int NL, int NP, int NTHR, typename V>
+__device__ __forceinline__ void cols_fetch(V *r, const V *__restrict__ T2, long long c0, long long ncol,
+                                           long long ntot, int tid)
+{
+    constexpr int TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int e = tid + t * NTHR;
+        const int l = e / N, p = e % N;
+        const int a = l / NP, q = l % NP;
+        if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = T2[(long long)a * ntot + (c0 + q) * N + p];
+    }
+}
+
+template <typename R, int N, int S>
+__global__ void __launch_bounds__(NTHR)
+    k_mid_otf_pipe(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
+                   int ntiles, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
+                   const int *__restrict__ done)
+{
+    typedef typename cvec<R>::type V;
+    if (*done) return;
+    constexpr int NP = mid_pairs<R, N, S>();
+    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+    PFDD_TILE(V, lds, N * LDP);
+    __shared__ R cst[S * 12];
+    __shared__ double red[NTHR / 64];
+    GConst<R> gc;
+    otf_consts<R, S>(cst, gc, G, threadIdx.x);
+    int t = blockIdx.x;
+    V r[PER];
+    {
+        const int mem = t / bpm;
+        cols_fetch<N, NL, NP, NTHR>(r, T2 + (long long)mem * S * ntot, (long long)(t - mem * bpm) * NP, ncol, ntot,
+                                    threadIdx.x);
+    }
+    double el = 0.0;
+    for (; t < ntiles; t += gridDim.x) {
+        // the thread index is re-read per tile so that per-element addresses
+        // are recomputed rather than held in registers across the loop
+        int tid = threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
+        const int mem = t / bpm;
+        const long long c0 = (long long)(t - mem * bpm) * NP;
+        V *T2m = T2 + (long long)mem * S * ntot;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = tid + k * NTHR;
+            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[k]);
+        }
+        __syncthreads();
+        const int tn = t + gridDim.x;
+        if (tn < ntiles) {
+            const int mn = tn / bpm;
+            cols_fetch<N, NL, NP, NTHR>(r, T2 + (long long)mn * S * ntot, (long long)(tn - mn * bpm) * NP, ncol,
+                                        ntot, tid);
+        }
+        el += otf_tile<R, N, S>(lds, tw, G, gc, cst, c0, ncol, tid);
+#pragma unroll 4
+        for (int e = tid; e < TOT; e += NTHR) {
+            const int l = e / N, p = e % N;
+            const int a = l / NP, q = l % NP;
+            if (c0 + q < ncol) T2m[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
+        }
+        __syncthreads();
+    }
+    double v[1] = {el};
+    block_sum<1, NTHR>(v, red, threadIdx.x);
+    if (threadIdx.x == 0) part_el[blockIdx.x] = v[0];
 }
 
 // ------------------------------------- pass 5: inverse z FFT + fused update
@@ -1156,7 +1255,8 @@ hipError_t launch_mid(int n, int S, int M, void *T2, const void *B, long long nc
 
 template <typename R, int S>
 hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long ncol, long long ntot,
-                            const void *tw, double *part_el, const int *done, hipStream_t st)
+                            const void *tw, double *part_el, const int *done, hipStream_t st, int pipe, int ncu,
+                            int *nblocks)
 {
     typedef typename cvec<R>::type V;
 #define PFDD_CASE(n_)                                                                             \
@@ -1164,8 +1264,24 @@ hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long 
         if constexpr (mid_fits<R, n_, S>()) {                                                     \
             constexpr int NP = mid_pairs<R, n_, S>();                                             \
             const int bpm = (int)((ncol + NP - 1) / NP);                                          \
-            PFDD_LAUNCH((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,          \
-                               (V *)T2, G, ncol, ntot, bpm, (const V *)tw, part_el, done);        \
+            if (!pipe) {                                                                          \
+                *nblocks = M * bpm;                                                               \
+                PFDD_LAUNCH((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,      \
+                            (V *)T2, G, ncol, ntot, bpm, (const V *)tw, part_el, done);           \
+            } else {                                                                              \
+                /* as many blocks as fit on the device at once */                                 \
+                static const int occ = [] {                                                       \
+                    int o = 0;                                                                    \
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
+                            &o, (const void *)k_mid_otf_pipe<R, n_, S>, NTHR, 0) != hipSuccess)   \
+                        o = 1;                                                                    \
+                    return o > 0 ? o : 1;                                                         \
+                }();                                                                              \
+                const long long nt = (long long)M * bpm, cap = (long long)ncu * occ;             \
+                *nblocks = (int)(nt < cap ? nt : cap);                                            \
+                PFDD_LAUNCH((k_mid_otf_pipe<R, n_, S>), dim3((unsigned)*nblocks), dim3(NTHR), st,  \
+                            (V *)T2, G, ncol, ntot, bpm, (int)nt, (const V *)tw, part_el, done);  \
+            }                                                                                     \
             return hipGetLastError();                                                             \
         } else {                                                                                  \
             return hipErrorInvalidValue;                                                          \
@@ -1178,15 +1294,16 @@ hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long 
 
 template <typename R>
 hipError_t launch_mid_otf(int n, int S, int M, void *T2, const GParams &G, long long ncol, long long ntot,
-                          const void *tw, double *part_el, const int *done, hipStream_t st)
+                          const void *tw, double *part_el, const int *done, hipStream_t st, int pipe, int ncu,
+                          int *nblocks)
 {
     switch (S) {
-    case 1: return launch_mid_otf_s<R, 1>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 2: return launch_mid_otf_s<R, 2>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 3: return launch_mid_otf_s<R, 3>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 4: return launch_mid_otf_s<R, 4>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 6: return launch_mid_otf_s<R, 6>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
-    case 12: return launch_mid_otf_s<R, 12>(n, M, T2, G, ncol, ntot, tw, part_el, done, st);
+    case 1: return launch_mid_otf_s<R, 1>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
+    case 2: return launch_mid_otf_s<R, 2>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
+    case 3: return launch_mid_otf_s<R, 3>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
+    case 4: return launch_mid_otf_s<R, 4>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
+    case 6: return launch_mid_otf_s<R, 6>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
+    case 12: return launch_mid_otf_s<R, 12>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
     }
     return hipErrorInvalidValue;
 }
@@ -1216,6 +1333,8 @@ struct pfdd_gl {
     long long pstride = 0;
     int nb3 = 0, nb5 = 0, nb5max = 0; // blocks of pass 3 / pass 5 (current tile, smallest tile)
     int tq[4] = {1, 1, 1, 1};          // tile shape of fwd z, fwd y, inv y, inv z + update
+    int xpipe = 0;                     // x pass: 0 one tile per block, 1 persistent pipelined
+    int ncu = 0;                       // compute units of the device
     double *stage2 = nullptr; // NSLOT * NB2
     unsigned *ticket = nullptr;
     bool stored_B = false;    // stream packed B(k) from HBM instead of evaluating it on the fly
@@ -1345,7 +1464,7 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
                               h->partial + 4 * h->pstride, h->done, st)));
     else
         HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, T2, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
-                                  h->partial + 4 * h->pstride, h->done, st)));
+                                  h->partial + 4 * h->pstride, h->done, st, h->xpipe, h->ncu, &h->nb3)));
     arm(3);
     HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
                                  h->tq[2])));
@@ -1370,24 +1489,26 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
 }
 
 // ---------------------------------------------------------------- autotune
-void set_tiles(pfdd_gl *h, const int tq[4])
+// tq[0..3]: tile shapes of the line passes, tq[4]: x-pass variant
+void set_tiles(pfdd_gl *h, const int tq[5])
 {
     for (int p = 0; p < 4; ++p) h->tq[p] = tq[p];
+    h->xpipe = h->stored_B ? 0 : tq[4];
     const long long n5 = (long long)h->SM * h->nx * h->ny;
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, tq[3]) : tile_lines_rt<float>(h->nz, tq[3]);
     h->nb5 = (int)((n5 + nl - 1) / nl);
     ++h->version; // captured graphs hold the old launch shapes
 }
 
-// Per-pass device time (ms) of the four tunable passes over nsteps GL
-// iterations with the handle's current tile choices.
+// Per-pass device time (ms) of the five tunable kernels over nsteps GL
+// iterations with the handle's current choices.
 template <typename R>
-int time_passes(pfdd_gl *h, int nsteps, double acc[4])
+int time_passes(pfdd_gl *h, int nsteps, double acc[5])
 {
     hipEvent_t ev[12];
     for (int q = 0; q < 12; ++q) HIPCHK(hipEventCreate(&ev[q]));
-    for (int p = 0; p < 4; ++p) acc[p] = 0.0;
-    const int pass_k[4] = {0, 1, 3, 4}; // fwd z, fwd y, inv y, inv z + update
+    for (int p = 0; p < 5; ++p) acc[p] = 0.0;
+    const int pass_k[5] = {0, 1, 3, 4, 2}; // fwd z, fwd y, inv y, inv z + update, x pass
     int rc = PFDD_OK;
     for (int s = 0; s < nsteps && !rc; ++s) {
         rc = step_once<R>(h, 0.0, ev);
@@ -1396,7 +1517,7 @@ int time_passes(pfdd_gl *h, int nsteps, double acc[4])
             rc = fail(PFDD_ERR_HIP, "autotune timing failed");
             break;
         }
-        for (int p = 0; p < 4; ++p) {
+        for (int p = 0; p < 5; ++p) {
             float t = 0;
             hipEventElapsedTime(&t, ev[2 * pass_k[p]], ev[2 * pass_k[p] + 1]);
             acc[p] += t;
@@ -1414,14 +1535,14 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::array<int, 4>> g_tune_cache;
+std::map<TuneKey, std::array<int, 5>> g_tune_cache;
 
-// Times every tile shape (0, 1, 2) of each line pass and keeps the fastest per
-// pass.  The field, ping-pong parity and step counters are restored.  Choices
-// that would break the reverse mode's one-system-per-block rule for the
-// update pass are skipped.
+// Times every tile shape (0, 1, 2) of each line pass and both x-pass variants
+// and keeps the fastest per kernel.  The field, ping-pong parity and step
+// counters are restored.  Choices that would break the reverse mode's
+// one-system-per-block rule for the update pass are skipped.
 template <typename R>
-int autotune(pfdd_gl *h, int nsteps, int choice[4])
+int autotune(pfdd_gl *h, int nsteps, int choice[5])
 {
     const TuneKey key{h->nx, h->ny, h->nz, h->S, h->M, h->prec,
                       (h->stored_B ? 1 : 0) | (h->mask ? 2 : 0) | (h->amp ? 4 : 0)};
@@ -1431,7 +1552,7 @@ int autotune(pfdd_gl *h, int nsteps, int choice[4])
         if (it != g_tune_cache.end()) {
             set_tiles(h, it->second.data());
             if (choice)
-                for (int p = 0; p < 4; ++p) choice[p] = it->second[p];
+                for (int p = 0; p < 5; ++p) choice[p] = it->second[p];
             return PFDD_OK;
         }
     }
@@ -1441,15 +1562,23 @@ int autotune(pfdd_gl *h, int nsteps, int choice[4])
     const int cur0 = h->cur;
     const bool stepped0 = h->stepped;
     HIPCHK(hipMemcpyAsync(save, h->xi[cur0], fb, hipMemcpyDeviceToDevice, h->stream));
-    double best[4] = {1e300, 1e300, 1e300, 1e300};
-    int arg[4] = {1, 1, 1, 1};
+    double best[5] = {1e300, 1e300, 1e300, 1e300, 1e300};
+    int arg[5] = {1, 1, 1, 1, 0};
     int rc = PFDD_OK;
-    for (int q = 0; q < 3 && !rc; ++q) {
-        const int all[4] = {q, q, q, q};
+    // candidates: tile shape q with the one-tile-per-block x pass, then the
+    // pipelined x pass (on-the-fly contraction only)
+    const int ncand = h->stored_B ? 3 : 4;
+    for (int c = 0; c < ncand && !rc; ++c) {
+        const int q = c < 3 ? c : arg[0];
+        const int all[5] = {q, q, q, q, c < 3 ? 0 : 1};
         set_tiles(h, all);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
-        double warm[4], t[4];
+        double warm[5], t[5];
         if ((rc = time_passes<R>(h, 1, warm)) || (rc = time_passes<R>(h, nsteps, t))) break;
+        if (c == 3) {
+            if (t[4] < best[4]) arg[4] = 1;
+            break;
+        }
         const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, q) : tile_lines_rt<float>(h->nz, q);
         const bool upd_ok = ((long long)h->nx * h->ny) % nl == 0;
         for (int p = 0; p < 4; ++p)
@@ -1457,6 +1586,7 @@ int autotune(pfdd_gl *h, int nsteps, int choice[4])
                 best[p] = t[p];
                 arg[p] = q;
             }
+        best[4] = t[4] < best[4] ? t[4] : best[4];
     }
     HIPCHK(hipMemcpyAsync(h->xi[cur0], save, fb, hipMemcpyDeviceToDevice, h->stream));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
@@ -1465,17 +1595,17 @@ int autotune(pfdd_gl *h, int nsteps, int choice[4])
     h->cur = cur0;
     h->stepped = stepped0;
     if (rc) {
-        const int dflt[4] = {1, 1, 1, 1};
+        const int dflt[5] = {1, 1, 1, 1, 0};
         set_tiles(h, dflt);
         return rc;
     }
     set_tiles(h, arg);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[key] = {arg[0], arg[1], arg[2], arg[3]};
+        g_tune_cache[key] = {arg[0], arg[1], arg[2], arg[3], arg[4]};
     }
     if (choice)
-        for (int p = 0; p < 4; ++p) choice[p] = arg[p];
+        for (int p = 0; p < 5; ++p) choice[p] = arg[p];
     return PFDD_OK;
 }
 
@@ -1495,7 +1625,7 @@ int adjoint_core(pfdd_gl *h, const void *xs, const void *lam_in, void *lam_out, 
                               h->partial + 4 * h->pstride, h->done, st)));
     else
         HIPCHK((launch_mid_otf<R>(h->nx, h->S, h->M, lam_out, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
-                                  h->partial + 4 * h->pstride, h->done, st)));
+                                  h->partial + 4 * h->pstride, h->done, st, h->xpipe, h->ncu, &h->nb3)));
     HIPCHK((launch_pass<R, true>(h->ny, lam_out, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
                                  h->tq[2])));
     const long long gstride = h->nb5max;
@@ -1902,6 +2032,9 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     h->N = (long long)d->nx * d->ny * d->nz;
     h->esz = d->precision == PFDD_FP64 ? 16 : 8;
     h->device = d->device;
+    if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess ||
+        h->ncu <= 0)
+        h->ncu = 256;
     for (int q = 0; q < 9; ++q) h->four[q] = d->four[q];
     for (int q = 0; q < 3 * d->ns; ++q) {
         h->xn[q] = d->xn[q];
@@ -2024,7 +2157,23 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     return PFDD_OK;
 }
 
-int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[4])
+int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5])
+{
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!choice) return fail(PFDD_ERR_ARG, "null choice");
+    for (int p = 0; p < 4; ++p)
+        if (choice[p] < 0 || choice[p] > 2) return fail(PFDD_ERR_ARG, "tile shape choice must be 0, 1 or 2");
+    if (choice[4] < 0 || choice[4] > 1) return fail(PFDD_ERR_ARG, "x-pass choice must be 0 or 1");
+    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, choice[3])
+                                        : tile_lines_rt<float>(h->nz, choice[3]);
+    if (h->tape.valid && ((long long)h->nx * h->ny) % nl != 0)
+        return fail(PFDD_ERR_UNSUPPORTED, "update tile shape incompatible with the recorded reverse-mode run");
+    set_tiles(h, choice);
+    return PFDD_OK;
+}
+
+int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5])
 {
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;

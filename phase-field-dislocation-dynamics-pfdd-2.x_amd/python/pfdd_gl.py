@@ -73,7 +73,7 @@ EXPORTS = [
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
     "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
-    "pfdd_gl_autotune",
+    "pfdd_gl_autotune", "pfdd_gl_set_tuning",
 ]
 
 _lib = None
@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_graph_run": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
         "pfdd_gl_graph_destroy": (c_int, [vp]),
         "pfdd_gl_autotune": (c_int, [vp, c_int, ip]),
+        "pfdd_gl_set_tuning": (c_int, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -229,11 +230,16 @@ class GLSolver:
         return r.as_dict() if sync else None
 
     def autotune(self, nsteps=4):
-        """Pick the fastest LDS tile shape per line pass; returns the choices
-        (fwd z, fwd y, inv y, inv z + update), each 0, 1 or 2."""
-        c = (ctypes.c_int * 4)()
+        """Pick the fastest LDS tile shape per line pass and the x-pass variant;
+        returns the choices (fwd z, fwd y, inv y, inv z + update: 0, 1 or 2;
+        x pass: 0 or 1)."""
+        c = (ctypes.c_int * 5)()
         _check(self._lib.pfdd_gl_autotune(self._h, int(nsteps), c))
         return list(c)
+
+    def set_tuning(self, choice):
+        c = (ctypes.c_int * 5)(*[int(v) for v in choice])
+        _check(self._lib.pfdd_gl_set_tuning(self._h, c))
 
     # --------------------------------------------------------- reverse mode
     def tape_record(self, nsteps, every):

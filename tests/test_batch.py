@@ -131,7 +131,7 @@ def test_autotune_keeps_results(precision):
     s1.step(3)
     x_before = s1.get_xi()
     choice = s1.autotune(3)
-    assert len(choice) == 4 and all(q in (0, 1, 2) for q in choice)
+    assert len(choice) == 5 and all(q in (0, 1, 2) for q in choice[:4]) and choice[4] in (0, 1)
     assert np.array_equal(s1.get_xi(), x_before)  # tuning restores the field
     s0.step(3)
     s0.step(17)
@@ -143,3 +143,22 @@ def test_autotune_keeps_results(precision):
     assert np.all(np.isfinite(lam)) and np.all(np.isfinite(dtau))
     s0.close()
     s1.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,shape,systems", [(0, (32, 32, 32), "basal3"), (1, (32, 32, 32), "basal3"),
+                                                     (0, (16, 16, 16), "fcc12"), (1, (48, 32, 16), "basal3")])
+def test_every_tuning_choice_gives_the_same_field(precision, shape, systems):
+    print(f"case: tuning choices {shape} {systems} precision {precision}", flush=True)
+    c = P.make_case(shape=shape, systems=systems, noise=1e-2, sigma_mu=0.004)
+    ref = P.gpu_solver(c, precision=precision)
+    ref.set_tuning([1, 1, 1, 1, 0])
+    ref.step(12)
+    x_ref = ref.get_xi()
+    ref.close()
+    for choice in ([0, 0, 0, 0, 0], [2, 2, 2, 2, 0], [1, 1, 1, 1, 1], [0, 2, 1, 0, 1]):
+        s = P.gpu_solver(c, precision=precision)
+        s.set_tuning(choice)
+        s.step(12)
+        assert np.array_equal(s.get_xi(), x_ref), choice
+        s.close()
