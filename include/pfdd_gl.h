@@ -49,11 +49,21 @@ enum pfdd_gl_precision { PFDD_FP64 = 0, PFDD_FP32 = 1 };
 enum pfdd_gl_core {
     PFDD_CORE_NONE = 0,
     PFDD_CORE_PERFECT = 1, /* flag 1, fft_fftw_slab.cpp:1130-1191: coef[0] = An'            */
+    PFDD_CORE_EXTENDED = 2,/* flag 2, :1243-1354: gamma surface coupling the 3 systems of
+                              each plane (ns % 3 == 0): coef[0..6] = c0..c4, a1, a3 over
+                              (mu dslip b)                                                  */
     PFDD_CORE_SINE = 3,    /* flag 3, :1360-1418: coef[0] = isf', coef[1] = Cn'             */
+    PFDD_CORE_PYRII = 4,   /* flag 4, :1429-1493 as executed: Fourier density
+                              aa0 + sum_m aa_m cos(2 pi m x) + bb_m sin(2 pi m x), m <= 4,
+                              coef[0..8] = aa0..aa4, bb1..bb4 over (mu dslip b); its force
+                              reaches only the last cell of each system (the reference's
+                              dE_core assignment follows the cell loop, :1491)             */
     PFDD_CORE_MPEA = 7,    /* flag 7, :1196-1235: amp[] = A_mpea/(mu dslip b) per cell      */
-    PFDD_CORE_SERIES = 8   /* sine series sum_m c_{a,m} sin^2(m pi x), optional amp[]:
+    PFDD_CORE_SERIES = 8,  /* sine series sum_m c_{a,m} sin^2(m pi x), optional amp[]:
                               coef[a*ncoef + m-1], generalises flags 1/3/7 (and 6, whose
                               An' the host recomputes per stress state)                   */
+    PFDD_CORE_FOURIER = 9  /* the Fourier density of flag 4 with its force on every cell
+                              (the model core_energy_pyrII describes); optional amp[]      */
 };
 
 typedef struct pfdd_gl pfdd_gl;

@@ -524,6 +524,26 @@ double orc_core_energy(int nx, int ny, int nz, int ns, const double *xi, int fla
                 ed = A * (sin(M_PI * x) * sin(M_PI * x));
                 E_ref += ed / nx;
                 d = A * M_PI * sin(2.0 * M_PI * x);
+            } else if (flag == 4 || flag == 9) {
+                /* core_energy_pyrII, :1429-1493: Fourier series
+                 * par = aa0..aa4, bb1..bb4 (already / (mu dslip b)).  Flag 4
+                 * is the routine as executed: its dE_core assignment sits after
+                 * the cell loop (:1491), so only the last cell of each system
+                 * gets the derivative (init_loop zeroes the rest) and E_core is
+                 * never accumulated.  Flag 9 applies the derivative to every
+                 * cell (the Fourier-series model the routine describes) and
+                 * accumulates E_core as flags 1/3/7 do. */
+                double c[5] = {0, 0, 0, 0, 0}, sn[5] = {0, 0, 0, 0, 0};
+                for (int m = 1; m <= 4; m++) {
+                    c[m] = cos(2.0 * m * M_PI * x);
+                    sn[m] = sin(2.0 * m * M_PI * x);
+                }
+                ed = par[0];
+                for (int m = 1; m <= 4; m++) ed += par[m] * c[m] + par[4 + m] * sn[m];
+                double dd = 0.0;
+                for (int m = 1; m <= 4; m++) dd += (2.0 * m * M_PI) * (par[4 + m] * c[m] - par[m] * sn[m]);
+                d = (flag == 9 || r == N - 1) ? dd : 0.0;
+                if (flag == 9) E_ref += ed / nx;
             } else if (flag == 8) {
                 double s = 0.0, e = 0.0;
                 for (int m = 1; m <= nm; m++) {
@@ -540,6 +560,61 @@ double orc_core_energy(int nx, int ny, int nz, int ns, const double *xi, int fla
             dE[index] = d;
             E_var += ed;
         }
+    if (flag == 2) {
+        /* core_energy_extended, :1243-1354: coupled gamma surface of the three
+         * systems of each plane (systems 3p, 3p+1, 3p+2); par = c0..c4, a1, a3
+         * (already / (mu dslip b)).  dE of system j is the derivative of the
+         * plane's density with respect to its Re xi, transcribed term by term
+         * from df1core/df2core/df3core; E_core is not accumulated (:1320). */
+        const double c0 = par[0], c1 = par[1], c2 = par[2], c3 = par[3], c4 = par[4], a1 = par[5], a3 = par[6];
+        const double tp = 2.0 * M_PI;
+        E_var = 0.0;
+        for (int pl = 0; pl < ns / 3; pl++)
+            for (long r = 0; r < N; r++) {
+                const long i1 = r + (3 * pl) * N, i2 = i1 + N, i3 = i2 + N;
+                const double x1 = xi[2 * i1], x2 = xi[2 * i2], x3 = xi[2 * i3];
+                const double f =
+                    c0 + c1 * (cos(tp * (x1 - x2)) + cos(tp * (x2 - x3)) + cos(tp * (x3 - x1))) +
+                    c2 * (cos(tp * (2 * x1 - x2 - x3)) + cos(tp * (2 * x2 - x3 - x1)) + cos(tp * (2 * x3 - x1 - x2))) +
+                    c3 * (cos(2 * tp * (x1 - x2)) + cos(2 * tp * (x2 - x3)) + cos(2 * tp * (x3 - x1))) +
+                    c4 * (cos(tp * (3 * x1 - x2 - 2 * x3)) + cos(tp * (3 * x1 - 2 * x2 - x3)) +
+                          cos(tp * (3 * x2 - x3 - 2 * x1)) + cos(tp * (3 * x2 - 2 * x3 - x1)) +
+                          cos(tp * (3 * x3 - x1 - 2 * x2)) + cos(tp * (3 * x3 - 2 * x1 - x2))) +
+                    a1 * (sin(tp * (x1 - x2)) + sin(tp * (x2 - x3)) + sin(tp * (x3 - x1))) +
+                    a3 * (sin(2 * tp * (x1 - x2)) + sin(2 * tp * (x2 - x3)) + sin(2 * tp * (x3 - x1)));
+                E_var += f;
+                dE[i1] = tp * (c1 * (sin(tp * (x2 - x1)) + sin(tp * (x3 - x1))) +
+                               c2 * (2.0 * sin(tp * (x2 + x3 - 2 * x1)) + sin(tp * (2 * x2 - x3 - x1)) +
+                                     sin(tp * (2 * x3 - x1 - x2))) +
+                               2.0 * c3 * (sin(2 * tp * (x2 - x1)) + sin(2 * tp * (x3 - x1))) +
+                               c4 * (3.0 * sin(tp * (x2 + 2 * x3 - 3 * x1)) + 3.0 * sin(tp * (2 * x2 + x3 - 3 * x1)) +
+                                     2.0 * sin(tp * (3 * x2 - x3 - 2 * x1)) + sin(tp * (3 * x2 - 2 * x3 - x1)) +
+                                     sin(tp * (3 * x3 - x1 - 2 * x2)) + 2.0 * sin(tp * (3 * x3 - 2 * x1 - x2))) +
+                               a1 * (cos(tp * (x1 - x2)) - cos(tp * (x3 - x1))) +
+                               2.0 * a3 * (cos(2 * tp * (x1 - x2)) - cos(2 * tp * (x3 - x1))));
+                dE[i2] = tp * (c1 * (sin(tp * (x1 - x2)) + sin(tp * (x3 - x2))) +
+                               c2 * (sin(tp * (2 * x1 - x2 - x3)) + 2.0 * sin(tp * (x3 + x1 - 2 * x2)) +
+                                     sin(tp * (2 * x3 - x1 - x2))) +
+                               2.0 * c3 * (sin(2 * tp * (x1 - x2)) + sin(2 * tp * (x3 - x2))) +
+                               c4 * (sin(tp * (3 * x1 - x2 - 2 * x3)) + 2.0 * sin(tp * (3 * x1 - 2 * x2 - x3)) +
+                                     3.0 * sin(tp * (x3 + 2 * x1 - 3 * x2)) + 3.0 * sin(tp * (2 * x3 + x1 - 3 * x2)) +
+                                     2.0 * sin(tp * (3 * x3 - x1 - 2 * x2)) + sin(tp * (3 * x3 - 2 * x1 - x2))) +
+                               a1 * (cos(tp * (x2 - x3)) - cos(tp * (x1 - x2))) +
+                               2.0 * a3 * (cos(2 * tp * (x2 - x3)) - cos(2 * tp * (x1 - x2))));
+                dE[i3] = tp * (c1 * (sin(tp * (x2 - x3)) + sin(tp * (x1 - x3))) +
+                               c2 * (sin(tp * (2 * x1 - x2 - x3)) + sin(tp * (2 * x2 - x3 - x1)) +
+                                     2.0 * sin(tp * (x1 + x2 - 2 * x3))) +
+                               2.0 * c3 * (sin(2 * tp * (x2 - x3)) + sin(2 * tp * (x1 - x3))) +
+                               c4 * (2.0 * sin(tp * (3 * x1 - x2 - 2 * x3)) + sin(tp * (3 * x1 - 2 * x2 - x3)) +
+                                     sin(tp * (3 * x2 - x3 - 2 * x1)) + 2.0 * sin(tp * (3 * x2 - 2 * x3 - x1)) +
+                                     3.0 * sin(tp * (x1 + 2 * x2 - 3 * x3)) + 3.0 * sin(tp * (2 * x1 + x2 - 3 * x3))) +
+                               a1 * (cos(tp * (x3 - x1)) - cos(tp * (x2 - x3))) +
+                               2.0 * a3 * (cos(2 * tp * (x3 - x1)) - cos(2 * tp * (x2 - x3))));
+            }
+        E_ref = 0.0;
+    } else if (flag == 4) {
+        E_ref = 0.0;
+    }
     if (e_var) *e_var = E_var;
     return E_ref;
 }

@@ -89,11 +89,48 @@ struct StepParams {
     double tau[MAXS];          // resolved shear per system (single-run handles)
     const double *taud;        // device tau per system of a batch (nullptr: use tau[])
     double coef[MAXS * MAXM];  // core coefficients per system of one member
+    double a0;                 // constant term of the Fourier core models
     double cddt;
     double invN;
+    long long ncell;           // cells per system
+    int model;                 // PFDD_CORE_*
     int ncoef;
     int S;                     // systems per member
 };
+
+// Coupled gamma surface of the three systems of one plane (core_energy_extended,
+// fft_fftw_slab.cpp:1243-1354): density f(y1, y2, y3) and its derivative with
+// respect to y1.  The surface is invariant under cyclic relabelling of the
+// three systems, so the derivative for system j is d1 of the cyclically
+// rotated triple (x_j, x_j+1, x_j+2).  c = c0..c4, a1, a3 over (mu dslip b).
+template <typename R>
+__device__ __forceinline__ void core_extended(R y1, R y2, R y3, const R *c, R &f, R &d1)
+{
+    const R arg[12] = {y1 - y2, y2 - y3, y3 - y1,                             // A1..A3
+                       2 * y1 - y2 - y3, 2 * y2 - y3 - y1, 2 * y3 - y1 - y2,  // B1..B3
+                       3 * y1 - y2 - 2 * y3, 3 * y1 - 2 * y2 - y3, 3 * y2 - y3 - 2 * y1,
+                       3 * y2 - 2 * y3 - y1, 3 * y3 - y1 - 2 * y2, 3 * y3 - 2 * y1 - y2}; // D1..D6
+    R sn[12], cs[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        if constexpr (sizeof(R) == 8) sincospi(2.0 * (double)arg[q], (double *)&sn[q], (double *)&cs[q]);
+        else sincospif(2.0f * (float)arg[q], (float *)&sn[q], (float *)&cs[q]);
+    }
+    R s2[3], c2[3]; // angles 4 pi A
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        s2[q] = (R)2 * sn[q] * cs[q];
+        c2[q] = cs[q] * cs[q] - sn[q] * sn[q];
+    }
+    const R c0 = c[0], c1 = c[1], cc2 = c[2], c3 = c[3], c4 = c[4], a1 = c[5], a3 = c[6];
+    f = c0 + c1 * (cs[0] + cs[1] + cs[2]) + cc2 * (cs[3] + cs[4] + cs[5]) + c3 * (c2[0] + c2[1] + c2[2]) +
+        c4 * (cs[6] + cs[7] + cs[8] + cs[9] + cs[10] + cs[11]) + a1 * (sn[0] + sn[1] + sn[2]) +
+        a3 * (s2[0] + s2[1] + s2[2]);
+    d1 = (R)(2.0 * M_PI) *
+         (c1 * (sn[2] - sn[0]) + cc2 * ((R)-2 * sn[3] + sn[4] + sn[5]) + (R)2 * c3 * (s2[2] - s2[0]) +
+          c4 * ((R)-3 * sn[6] - (R)3 * sn[7] + (R)2 * sn[8] + sn[9] + sn[10] + (R)2 * sn[11]) +
+          a1 * (cs[0] - cs[2]) + (R)2 * a3 * (c2[0] - c2[2]));
+}
 
 struct BParams {
     double sig[MAXS * 9]; // sigma_a = C : eps_a
@@ -636,6 +673,7 @@ __global__ void __launch_bounds__(NTHR)
     // per-line tau and core coefficients: read once per block here, since
     // kernel-argument arrays indexed per lane compile to per-lane memory loads
     __shared__ R ltau[NL], lcoef[MAXM * NL];
+    __shared__ int lsub[NL]; // position of the line's system in its plane triple (coupled model)
     __shared__ double red[4 * (NTHR / 64)];
     const int tid = threadIdx.x;
     const long long L0 = (long long)blockIdx.x * NL;
@@ -648,6 +686,7 @@ __global__ void __launch_bounds__(NTHR)
         const int as = a % P.S;                           // system within its member
         ltau[l] = (R)(P.taud ? P.taud[a] : P.tau[a]);
         for (int m = 0; m < MAXM; ++m) lcoef[m * NL + l] = (R)P.coef[as * MAXM + m];
+        lsub[l] = as % 3;
     }
     __syncthreads();
     tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
@@ -690,23 +729,60 @@ __global__ void __launch_bounds__(NTHR)
         const V g = lds.ld(p * LDP + l);
         V x;
         x = xold[t];
-        // core energy density and its derivative at Re xi (sine series)
-        R sp, cp;
-        if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
-        else sincospif((float)x.x, (float *)&sp, (float *)&cp);
-        R sm = sp, cm = cp, dens = (R)0, dE = (R)0;
-        for (int m = 1; m <= nm; ++m) {
-            const R c = lcoef[(m - 1) * NL + l];
-            dens += c * sm * sm;
-            dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
-            const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
-            sm = sn;
-            cm = cn;
+        // core energy density and its derivative at Re xi of the old field
+        R dens = (R)0, dE = (R)0;
+        if (P.model == PFDD_CORE_EXTENDED) {
+            // the other two systems of the plane at the same cell
+            const int j = lsub[l];
+            const long long b0 = idx - (long long)j * P.ncell;
+            R y[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) y[q] = q == j ? x.x : xo[b0 + q * P.ncell].x;
+            R cc[7];
+#pragma unroll
+            for (int q = 0; q < 7; ++q) cc[q] = lcoef[q * NL + l];
+            R f;
+            core_extended<R>(y[j], y[(j + 1) % 3], y[(j + 2) % 3], cc, f, dE);
+            dens = f / (R)3; // each of the three systems carries a third of the plane's density
+        } else {
+            R sp, cp;
+            if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
+            else sincospif((float)x.x, (float *)&sp, (float *)&cp);
+            if (P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER) {
+                // a0 + sum_m a_m cos(2 pi m x) + b_m sin(2 pi m x), m <= 4
+                const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
+                R sm = s1, cm = c1;
+                dens = (R)P.a0;
+#pragma unroll
+                for (int m = 1; m <= 4; ++m) {
+                    const R am = lcoef[(m - 1) * NL + l], bm = lcoef[(m + 3) * NL + l];
+                    dens += am * cm + bm * sm;
+                    dE += (R)(2.0 * M_PI * m) * (bm * cm - am * sm);
+                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
+                    sm = sn;
+                    cm = cn;
+                }
+                // core_energy_pyrII as executed: its dE_core assignment follows
+                // the cell loop (fft_fftw_slab.cpp:1491), so only each system's
+                // last cell gets the force
+                if (P.model == PFDD_CORE_PYRII && (idx % P.ncell) != P.ncell - 1) dE = (R)0;
+            } else {
+                // sine series sum_m c_m sin^2(m pi x) (flags 1, 3, 7, 8)
+                R sm = sp, cm = cp;
+                for (int m = 1; m <= nm; ++m) {
+                    const R c = lcoef[(m - 1) * NL + l];
+                    dens += c * sm * sm;
+                    dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
+                    const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
+                    sm = sn;
+                    cm = cn;
+                }
+            }
+            R A = (R)1;
+            if (amp) A = (R)amp[idx];
+            dens *= A;
+            dE *= A;
         }
-        R A = (R)1;
-        if (amp) A = (R)amp[idx];
-        dens *= A;
-        dE *= A;
         const R tau = ltau[l];
         V y;
         double dn;
@@ -1472,8 +1548,11 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
                             (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done, st));
     arm(5);
+    // E_core as the reference accumulates it (sum / nx); core_energy_extended
+    // and core_energy_pyrII do not accumulate it (fft_fftw_slab.cpp:1320,1483)
+    const bool no_eref = P.model == PFDD_CORE_EXTENDED || P.model == PFDD_CORE_PYRII;
     PFDD_LAUNCH(k_reduce, dim3(NB2), dim3(NTHR), st, h->partial, h->pstride, h->nb5, h->nb3, 1.0 / (double)h->N,
-                1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done, h->steps);
+                no_eref ? 0.0 : 1.0 / (double)h->nx, tol, h->stage2, h->ticket, h->res, h->done, h->steps);
     HIPCHK(hipGetLastError());
     return PFDD_OK;
 }
@@ -1672,6 +1751,8 @@ int set_coef(pfdd_gl *h, int model, int ncoef, const double *coef, bool has_amp)
 {
     StepParams &P = h->P;
     for (int q = 0; q < MAXS * MAXM; ++q) P.coef[q] = 0.0;
+    P.a0 = 0.0;
+    P.model = model;
     switch (model) {
     case PFDD_CORE_NONE:
         P.ncoef = 0;
@@ -1693,6 +1774,21 @@ int set_coef(pfdd_gl *h, int model, int ncoef, const double *coef, bool has_amp)
         if (!has_amp) return fail(PFDD_ERR_ARG, "core model 7 needs the per-cell amplitude amp[]");
         P.ncoef = 1;
         for (int a = 0; a < h->S; ++a) P.coef[a * MAXM] = 1.0;
+        break;
+    case PFDD_CORE_EXTENDED:
+        if (!coef) return fail(PFDD_ERR_ARG, "core model 2 needs coef[0..6] = c0..c4, a1, a3");
+        if (h->S % 3) return fail(PFDD_ERR_UNSUPPORTED, "core model 2 couples systems in threes (ns %% 3 == 0)");
+        P.ncoef = 7;
+        for (int a = 0; a < h->S; ++a)
+            for (int m = 0; m < 7; ++m) P.coef[a * MAXM + m] = coef[m];
+        break;
+    case PFDD_CORE_PYRII:
+    case PFDD_CORE_FOURIER:
+        if (!coef) return fail(PFDD_ERR_ARG, "core models 4/9 need coef[0..8] = aa0..aa4, bb1..bb4");
+        P.ncoef = 8;
+        P.a0 = coef[0];
+        for (int a = 0; a < h->S; ++a)
+            for (int m = 0; m < 8; ++m) P.coef[a * MAXM + m] = coef[1 + m];
         break;
     case PFDD_CORE_SERIES:
         if (ncoef < 0 || ncoef > MAXM) return fail(PFDD_ERR_ARG, "ncoef must be in [0, %d]", MAXM);
@@ -2053,6 +2149,7 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     h->xnu = young / 2.0 / h->mu - 1.0;
     h->stored_B = (d->flags & PFDD_GL_FLAG_STORED_B) != 0;
     h->P.S = d->ns;
+    h->P.ncell = h->N;
     h->P.cddt = d->cddt;
     h->P.invN = 1.0 / (double)h->N;
 
@@ -2281,6 +2378,8 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
     if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
+    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_PYRII || h->P.model == PFDD_CORE_FOURIER)
+        return fail(PFDD_ERR_UNSUPPORTED, "reverse mode supports the sine-series core models (0, 1, 3, 7, 8)");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3]) : tile_lines_rt<float>(h->nz, h->tq[3]);
     if (((long long)h->nx * h->ny) % nl != 0)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);

@@ -209,24 +209,22 @@ void Output::stats(App &app, FFT_HIP &fft, int timeflag)
 
 void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
 {
-    // DumpText::write_header (dump_text.cpp:363-371) + per-site columns
+    // Dump::write (dump.cpp:145-233) for one process: the file is created at
+    // the first snapshot and appended to afterwards.  Text: DumpText::header_text
+    // (dump_text.cpp:361-372) + one line per site; binary (.bin):
+    // DumpText::header_binary (:341-357) -- idump, time, ndump, box bounds in
+    // lattice units, size_one, nprocs = 1 -- then write_binary (:376-381):
+    // the count of doubles and the site-major array.
     fft.sync_host();
-    FILE *fp = fopen(d.file.c_str(), "a");
+    FILE *fp = fopen(d.file.c_str(), d.idump == 0 ? (d.binary ? "wb" : "w") : (d.binary ? "ab" : "a"));
     if (!fp) deck_error("Cannot open dump file " + d.file);
     const long long N = (long long)fft.nx * fft.ny * fft.nz;
-    static int idump = 0;
-    std::string cols;
-    for (auto &f : d.fields) cols += f + " ";
-    if (!cols.empty()) cols.pop_back();
-    fprintf(fp, "ITEM: TIMESTEP\n%d %10g\nITEM: NUMBER OF ATOMS\n%lld\nITEM: BOX BOUNDS\n", idump++, time, N);
-    fprintf(fp, "%g %g\n%g %g\n%g %g\n", 0.0, app.scale[0] * fft.nx, 0.0, app.scale[1] * fft.ny, 0.0,
-            app.scale[2] * fft.nz);
-    fprintf(fp, "ITEM: ATOMS %s\n", cols.c_str());
+    const int size_one = (int)d.fields.size();
+    std::vector<double> buf((size_t)N * size_one);
     for (long long r = 0; r < N; ++r) {
         const int i = (int)(r / ((long long)fft.ny * fft.nz)), j = (int)((r / fft.nz) % fft.ny), k = (int)(r % fft.nz);
-        std::string line;
-        char buf[64];
-        for (auto &f : d.fields) {
+        for (int c = 0; c < size_one; ++c) {
+            const std::string &f = d.fields[c];
             double v = 0.0;
             if (f == "x") v = i;
             else if (f == "y") v = j;
@@ -238,16 +236,46 @@ void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
                 // AppFCC::compute_stress / compute_strain (app_fcc.cpp:438-502): component
                 // (i, j) of the last computed data_sigma / data_eps.  As in the reference,
                 // dumps are written before the diagnostics of the same output step.
-                const int i = f[1] - 'x', j = f[2] - 'x';
+                const int ii = f[1] - 'x', jj = f[2] - 'x';
                 const std::vector<double> &src = f[0] == 'p' ? fft.data_sigma : fft.data_eps;
-                v = src.empty() ? 0.0 : src[2 * (r + (long long)i * N + (long long)j * 3 * N)];
+                v = src.empty() ? 0.0 : src[2 * (r + (long long)ii * N + (long long)jj * 3 * N)];
             }
-            snprintf(buf, sizeof buf, "%g ", v);
-            line += buf;
+            buf[(size_t)r * size_one + c] = v;
         }
-        if (!line.empty()) line.pop_back();
-        fprintf(fp, "%s\n", line.c_str());
     }
+    if (d.binary) {
+        const int ndump = (int)N, nprocs = 1;
+        const double box[6] = {0.0, (double)fft.nx, 0.0, (double)fft.ny, 0.0, (double)fft.nz};
+        fwrite(&d.idump, sizeof(int), 1, fp);
+        fwrite(&time, sizeof(double), 1, fp);
+        fwrite(&ndump, sizeof(int), 1, fp);
+        fwrite(box, sizeof(double), 6, fp);
+        fwrite(&size_one, sizeof(int), 1, fp);
+        fwrite(&nprocs, sizeof(int), 1, fp);
+        const int n = ndump * size_one;
+        fwrite(&n, sizeof(int), 1, fp);
+        fwrite(buf.data(), sizeof(double), buf.size(), fp);
+    } else {
+        std::string cols;
+        for (auto &f : d.fields) cols += f + " ";
+        if (!cols.empty()) cols.pop_back();
+        fprintf(fp, "ITEM: TIMESTEP\n%d %10g\nITEM: NUMBER OF ATOMS\n%lld\nITEM: BOX BOUNDS\n", d.idump, time, N);
+        fprintf(fp, "%g %g\n%g %g\n%g %g\n", 0.0, app.scale[0] * fft.nx, 0.0, app.scale[1] * fft.ny, 0.0,
+                app.scale[2] * fft.nz);
+        fprintf(fp, "ITEM: ATOMS %s\n", cols.c_str());
+        // DumpText::write_text (:385-400): "%g " per value, then a newline
+        std::string line;
+        char num[64];
+        for (long long r = 0; r < N; ++r) {
+            line.clear();
+            for (int c = 0; c < size_one; ++c) {
+                snprintf(num, sizeof num, "%g ", buf[(size_t)r * size_one + c]);
+                line += num;
+            }
+            fprintf(fp, "%s\n", line.c_str());
+        }
+    }
+    ++d.idump;
     fclose(fp);
 }
 
@@ -559,6 +587,7 @@ void Input::command(const std::vector<std::string> &w)
         d.id = w[1];
         d.every = num(w, 3, "dump");
         d.file = w[4];
+        d.binary = d.file.size() > 4 && d.file.compare(d.file.size() - 4, 4, ".bin") == 0;
         for (size_t i = 5; i < w.size(); ++i) {
             const std::string &f = w[i];
             static const char *tens[12] = {"pxx", "pyy", "pzz", "pxy", "pxz", "pyz",

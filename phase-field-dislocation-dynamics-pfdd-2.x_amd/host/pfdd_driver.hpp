@@ -81,6 +81,8 @@ struct DumpSpec {
     double every = 1;
     std::vector<std::string> fields;
     double next_time = 0;
+    int idump = 0;       // snapshots written (Dump::idump, dump.cpp:114,172)
+    bool binary = false; // file name ends in .bin (dump.cpp:81-82)
 };
 
 class Output {

@@ -143,6 +143,18 @@ def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigm
             par[:, 2:] = 0.1 * base[0] * (2 * rng.random((S, ncoef - 2)) - 1)
         flag, nm = 8, ncoef
         gpu_core = dict(core_model=8, coef=par.reshape(-1), ncoef=ncoef)
+    elif core == "extended":
+        # Cu gamma-surface constants of material.cpp (c0..c4, a1, a3, J/m^2)
+        mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
+        par = np.array([344.948e-3, -85.399e-3, -40.421e-3, 12.396e-3, -0.674e-3, -131.393e-3, -19.816e-3]) / mdb
+        flag, nm = 2, 0
+        gpu_core = dict(core_model=2, coef=par, ncoef=0)
+    elif core in ("pyrII", "fourier"):
+        # a pyramidal-II-like asymmetric GSFE: aa0..aa4, bb1..bb4 (J/m^2)
+        mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
+        par = np.array([216.6e-3, -150.0e-3, -45.0e-3, -15.0e-3, -6.6e-3, -51.94e-3, 30.0e-3, -8.0e-3, 2.0e-3]) / mdb
+        flag, nm = (4 if core == "pyrII" else 9), 0
+        gpu_core = dict(core_model=flag, coef=par, ncoef=0)
     elif core == "none":
         flag, par, nm = 0, np.zeros(1), 0
         gpu_core = dict(core_model=0, coef=None, ncoef=0)

@@ -160,3 +160,60 @@ def test_deck_run_matches_oracle(tmp_path, max_iter, tol, run):
     err = np.max(np.abs(xi - ref)) / np.max(np.abs(ref))
     assert err < 1e-9, err
     assert "Loop time of" in r.stdout
+
+
+def read_text_dumps(path):
+    snaps, lines = [], open(path).read().splitlines()
+    q = 0
+    while q < len(lines):
+        assert lines[q] == "ITEM: TIMESTEP"
+        idump, t = lines[q + 1].split()
+        n = int(lines[q + 3])
+        cols = lines[q + 8].split()[2:]
+        rows = np.array([[float(v) for v in lines[q + 9 + r].split()] for r in range(n)])
+        assert lines[q + 9].endswith(" ")  # "%g " per value, as DumpText::write_text
+        snaps.append((int(idump), float(t), cols, rows))
+        q += 9 + n
+    return snaps
+
+
+def read_bin_dumps(path):
+    raw, off, snaps = open(path, "rb").read(), 0, []
+    while off < len(raw):
+        idump, = np.frombuffer(raw, np.int32, 1, off); off += 4
+        t, = np.frombuffer(raw, np.float64, 1, off); off += 8
+        ndump, = np.frombuffer(raw, np.int32, 1, off); off += 4
+        box = np.frombuffer(raw, np.float64, 6, off); off += 48
+        size_one, nprocs = np.frombuffer(raw, np.int32, 2, off); off += 8
+        n, = np.frombuffer(raw, np.int32, 1, off); off += 4
+        assert n == ndump * size_one and nprocs == 1
+        data = np.frombuffer(raw, np.float64, n, off).reshape(ndump, size_one); off += 8 * n
+        snaps.append((int(idump), float(t), box, data))
+    return snaps
+
+
+@pytest.mark.gpu
+def test_deck_text_and_binary_dumps(tmp_path):
+    print("case: driver deck 16^3 x 3 with text and .bin dumps every 5 of 10 steps", flush=True)
+    fields = "x y z xir1 xii1 xir2 pxz exz pxx"
+    extra = (f"dump 1 text 5 d.txt {fields}\n"
+             f"dump 2 text 5 d.bin {fields}\n"
+             "write_xi xi_final.bin\n")
+    text = deck_text(max_iter=1, tol=0, run=10, s0="0 0 0 0 0.002 0", ds="0 0 0 0 0 0", noise=1e-2, extra=extra)
+    r = run_exe(text, tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ts = read_text_dumps(tmp_path / "d.txt")
+    bs = read_bin_dumps(tmp_path / "d.bin")
+    # with max_iter 1 every stress state ends in an output event that writes
+    # all dumps (SolveGL::iterate, solve_GL.cpp:108): setup + 10 snapshots
+    assert len(ts) == len(bs) == 11
+    for q, ((i1, t1, cols, rows), (i2, t2, box, data)) in enumerate(zip(ts, bs)):
+        assert i1 == i2 == q and t1 == pytest.approx(t2)
+        assert cols == fields.split()
+        assert np.allclose(box, [0, 16, 0, 16, 0, 16])
+        assert np.allclose(rows, data, rtol=1e-5, atol=1e-300 + 1e-5 * np.abs(data).max(axis=0))
+    xi = np.fromfile(tmp_path / "xi_final.bin", dtype=np.float64).view(np.complex128).reshape(3, 16 ** 3)
+    last = bs[-1][3]
+    assert np.array_equal(last[:, 3], xi[0].real) and np.array_equal(last[:, 4], xi[0].imag)
+    assert np.array_equal(last[:, 5], xi[1].real)
+    assert np.array_equal(last[:, 0], np.repeat(np.arange(16.0), 256))

@@ -224,3 +224,26 @@ def test_full_size_c1_properties_fp64():
         assert relerr(x2, np.roll(x1, shift, axis=(1, 2, 3))) < 1e-12
     finally:
         g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("core,systems,precision", [("extended", "basal3", 0), ("extended", "fcc12", 0),
+                                                    ("pyrII", "basal3", 0), ("fourier", "basal3", 0),
+                                                    ("extended", "basal3", 1), ("fourier", "basal3", 1)])
+def test_other_core_models_match_oracle(core, systems, precision):
+    print(f"case: core model {core}, 16^3 {systems}, precision {precision}, 20 steps", flush=True)
+    case = P.make_case(shape=(16, 16, 16), systems=systems, core=core, noise=2e-2, sigma_mu=0.004,
+                       loops=[(0, 2, 8, 8, 8, 4)])
+    st = P.oracle_state(case)
+    s = P.gpu_solver(case, precision=precision)
+    for _ in range(20):
+        ref = st.step(case["tau"])
+    r = s.step(20)
+    x = s.get_xi()
+    s.close()
+    tol = 1e-9 if precision == 0 else 1e-5
+    err = np.max(np.abs(x - st.xi)) / np.max(np.abs(st.xi))
+    assert err < tol, err
+    assert abs(r["e_core"] - ref["E_core"]) <= tol * max(1.0, abs(ref["E_core"])) * 10
+    assert abs(r["e_core_ref"] - ref["E_core_ref"]) <= tol * max(1.0, abs(ref["E_core_ref"]))
+    assert abs(r["e_total"] - ref["E_total"]) <= tol * max(1.0, abs(ref["E_total"])) * 10

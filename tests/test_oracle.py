@@ -179,6 +179,68 @@ def test_core_energy_sine_closed_form():
     assert abs(ev - dens.sum()) < 1e-15 and abs(eref - dens.sum() / 1) < 1e-15  # nx = 1
 
 
+def _elementwise_fd(xi, flag, par, picks, h=1e-6):
+    out = []
+    for (a, r) in picks:
+        xp, xm = xi.copy(), xi.copy()
+        xp.reshape(xi.shape[0], -1)[a, r] += h
+        xm.reshape(xi.shape[0], -1)[a, r] -= h
+        _, _, ep = oracle.core_energy(xp, flag, par)
+        _, _, em = oracle.core_energy(xm, flag, par)
+        out.append((ep - em) / (2 * h))
+    return np.array(out)
+
+
+def test_core_energy_extended_pins():
+    """core_energy_extended (fft_fftw_slab.cpp:1243-1354), coupled gamma surface
+    of the 3 systems of each plane: finite differences of the plane density
+    per element, cyclic symmetry of the three systems, closed form at zero
+    field, integer-shift invariance, E_core not accumulated."""
+    rng = np.random.default_rng(5)
+    par = rng.normal(size=7) * 0.01
+    xi = (rng.random((6, 4, 4, 4)) * 2 - 0.5).astype(complex)
+    dE, eref, ev = oracle.core_energy(xi, 2, par)
+    assert eref == 0.0
+    picks = [(a, int(r)) for a in range(6) for r in rng.integers(0, 64, 3)]
+    fd = _elementwise_fd(xi, 2, par, picks)
+    got = np.array([dE.reshape(6, -1)[a, r] for a, r in picks])
+    assert np.allclose(fd, got, rtol=1e-7, atol=1e-9 * np.abs(got).max())
+    # cyclic relabelling of the systems of each plane relabels dE
+    perm = [1, 2, 0, 4, 5, 3]
+    dEp, _, evp = oracle.core_energy(xi[perm], 2, par)
+    assert np.allclose(dEp, dE[perm], rtol=1e-13, atol=1e-15) and abs(evp - ev) < 1e-12 * abs(ev)
+    # zero field: density c0 + 3 c1 + 3 c2 + 3 c3 + 6 c4 per plane and cell, no force
+    dE0, _, ev0 = oracle.core_energy(np.zeros_like(xi), 2, par)
+    c0, c1, c2, c3, c4 = par[:5]
+    assert abs(ev0 - 2 * 64 * (c0 + 3 * c1 + 3 * c2 + 3 * c3 + 6 * c4)) < 1e-12
+    assert np.all(np.abs(dE0) < 1e-15)
+    dE1, _, _ = oracle.core_energy(xi + 1.0, 2, par)
+    assert np.allclose(dE1, dE, rtol=1e-9, atol=1e-12)
+
+
+def test_core_energy_fourier_and_pyrII_as_executed():
+    """core_energy_pyrII (:1429-1493): flag 9 = the Fourier series at every
+    cell (finite differences, closed form at 0); flag 4 = the routine as
+    executed, whose misplaced dE_core assignment (:1491) leaves the force only
+    on the last cell of each system, with E_core not accumulated."""
+    rng = np.random.default_rng(9)
+    par = rng.normal(size=9) * 0.01
+    xi = (rng.random((2, 4, 4, 4)) * 2 - 0.5).astype(complex)
+    d9, e9ref, ev9 = oracle.core_energy(xi, 9, par)
+    picks = [(a, int(r)) for a in range(2) for r in rng.integers(0, 64, 4)]
+    fd = _elementwise_fd(xi, 9, par, picks)
+    got = np.array([d9.reshape(2, -1)[a, r] for a, r in picks])
+    assert np.allclose(fd, got, rtol=1e-7, atol=1e-9 * np.abs(got).max())
+    d0, _, ev0 = oracle.core_energy(np.zeros_like(xi), 9, par)
+    m = np.arange(1, 5)
+    assert abs(ev0 - 128 * (par[0] + par[1:5].sum())) < 1e-12
+    assert np.allclose(d0, 2 * np.pi * np.sum(m * par[5:9]), rtol=1e-14)
+    d4, e4ref, ev4 = oracle.core_energy(xi, 4, par)
+    assert e4ref == 0.0 and ev4 == ev9
+    flat4, flat9 = d4.reshape(2, -1), d9.reshape(2, -1)
+    assert np.all(flat4[:, :-1] == 0.0) and np.array_equal(flat4[:, -1], flat9[:, -1])
+
+
 # ------------------------------------------------------------ the full step
 def _energy(case, xi):
     st = P.oracle_state(dict(case, xi0=xi), case["_BB"])
