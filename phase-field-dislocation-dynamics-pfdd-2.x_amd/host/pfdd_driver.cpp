@@ -74,6 +74,12 @@ void FFT_HIP::init()
         d.amp = amp.data();
     } else if (core_flag == 8) {
         d.core_model = PFDD_CORE_SERIES;
+    } else if (core_flag == 2) {
+        d.core_model = PFDD_CORE_EXTENDED;
+    } else if (core_flag == 4) {
+        d.core_model = PFDD_CORE_PYRII;
+    } else if (core_flag == 9) {
+        d.core_model = PFDD_CORE_FOURIER;
     } else if (core_flag == 0) {
         d.core_model = PFDD_CORE_NONE;
     } else {
@@ -329,10 +335,12 @@ void App::init_app(FFT_HIP &fft)
     fft.CD = CD >= 0 ? CD : fft.slip.CD;
     fft.timestep = timestep >= 0 ? timestep : fft.slip.timestep;
     fft.core_flag = core_flag >= 0 ? core_flag : fft.slip.default_core;
-    if (fft.core_flag == 1 || fft.core_flag == 3) {
+    if (fft.core_flag == 1 || fft.core_flag == 2 || fft.core_flag == 3 || fft.core_flag == 4 || fft.core_flag == 9) {
         fft.coef = core_coefficients(fft.mat, fft.core_flag);
         fft.ncoef = (int)fft.coef.size();
     }
+    if (fft.core_flag == 2 && fft.slip_systems % 3)
+        deck_error("core_energy 2 couples the slip systems of each plane in threes");
     if (fft.core_flag == 7 && fft.mat.A_mpea.empty()) deck_error("core_energy 7 needs 'material ... mpea FILE'");
     Grid g{fft.nx, fft.ny, fft.nz, fft.slip_systems};
     const long long n = (long long)g.S * g.N();
@@ -538,6 +546,9 @@ void Input::command(const std::vector<std::string> &w)
                 else if (k == "c4") m.c4 = v;
                 else if (k == "a1") m.a1 = v;
                 else if (k == "a3") m.a3 = v;
+                else if (k.size() == 3 && (k.compare(0, 2, "aa") == 0 || k.compare(0, 2, "bb") == 0) &&
+                         k[2] >= '0' && k[2] <= '4' && !(k[0] == 'b' && k[2] == '0'))
+                    m.pyr[k[0] == 'a' ? k[2] - '0' : 4 + (k[2] - '0')] = v;
                 else deck_error("Illegal material command: " + k);
             }
         }

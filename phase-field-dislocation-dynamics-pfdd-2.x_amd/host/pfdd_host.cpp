@@ -11,35 +11,113 @@ namespace pfdd_host {
 // ------------------------------------------------------------------ Material
 bool Material::preset(const std::string &name)
 {
-    // set_constants, material.cpp:176-276 (elastic and GSF part of the presets)
-    if (name == "Ni") {
-        a = 3.52E-10;
-        b = a / std::sqrt(2.0);
-    } else if (name == "Cu") {
-        a = 3.64E-10;
-        b = a / std::sqrt(2.0);
-        mu = 54.5E9;
-        young = 144.4E9;
-        c0 = 344.948E-3;
-        c1 = -85.399E-3;
-        c2 = -40.421E-3;
-        c3 = 12.396E-3;
-        c4 = -0.674E-3;
-        a1 = -131.393E-3;
-        a3 = -19.816E-3;
-        isf = 38.5E-3;
-        usf = 163.7E-3;
-    } else if (name == "Pd") {
-        a = 3.950E-10;
-        b = a / std::sqrt(2.0);
-        mu = 45.7E9;
-        young = 134.60E9;
-        usf = 0.76308;
-        An = usf;
-    } else {
-        return false;
+    // set_constants, material.cpp:176-835: elastic constants, Burgers vector
+    // rule, sine-core (isf, usf, An), extended gamma-surface (c0..c4, a1, a3)
+    // and pyramidal-II Fourier (aa0..aa4, bb1..bb4) constants of each named
+    // preset.  Conductivity constants are not part of the GL path.
+    // b: 0 = a/sqrt2 (fcc), 1 = sqrt3 a/2 (bcc), 2 = given, 3 = not set
+    struct Row {
+        const char *name;
+        double a;
+        int brule;
+        double b, mu, young;
+        double c[7]; // c0..c4, a1, a3
+        double isf, usf, An;
+        double pyr[9]; // aa0..aa4, bb1..bb4
+    };
+    static const double MG_PYR[9] = {216.6E-3, -82.84E-3, -116.8E-3, -5.873E-3, -11.27E-3,
+                                     -51.94E-3, 22.04E-3, 9.44E-3, -0.2501E-3};
+    static const Row rows[] = {
+        {"Ni", 3.52E-10, 0, 0, 99.7E9, 255.7E9,
+         {503.144E-3, -106.533E-3, -83.741E-3, 28.777E-3, -2.916E-3, -123.917E-3, -38.533E-3}, 144.5E-3, 289.0E-3, 0, {}},
+        {"Al", 4.04E-10, 0, 0, 28.4E9, 76.0E9,
+         {149.56346E-3, -4.23464E-3, -74.39331E-3, 33.84682E-3, -2.39438E-3, 67.75268E-3, -30.88995E-3}, 140.2E-3,
+         177.0E-3, 0, {}},
+        {"Au", 4.17E-10, 0, 0, 18.8E9, 54.0E9,
+         {161.832E-3, -40.659E-3, -17.320E-3, 3.197E-3, 0.472E-3, -64.718E-3, -13.282E-3}, 27.9E-3, 66.5E-3, 0, {}},
+        {"Cu", 3.64E-10, 0, 0, 54.5E9, 144.4E9,
+         {344.948E-3, -85.399E-3, -40.421E-3, 12.396E-3, -0.674E-3, -131.393E-3, -19.816E-3}, 38.5E-3, 163.7E-3, 0, {}},
+        {"Pd", 3.950E-10, 0, 0, 45.7E9, 125.8E9,
+         {308.65990E-3, -63.66363E-3, -48.76003E-3, 12.024463E-3, -1.11869E-3, -61.34161E-3, -23.33224E-3}, 138.1E-3,
+         197.9E-3, 0, {}},
+        {"Ag", 4.16E-10, 0, 0, 27.5E9, 74.8E9,
+         {179.573E-3, -32.837E-3, -39.578E-3, 14.700E-3, -0.989E-3, -43.791E-3, -17.254E-3}, 17.8E-3, 100.4E-3, 0, {}},
+        {"Pt", 3.98E-10, 0, 0, 53.2E9, 149.0E9,
+         {223.27233E-3, -15.38077E-3, -75.36747E-3, 21.70531E-3, -2.52876E-3, 58.60391E-3, -43.19019E-3}, 253.7E-3,
+         257.5E-3, 0, {}},
+        {"Rh", 3.84E-10, 0, 0, 154.1E9, 384.9E9,
+         {556.35590E-3, -46.42878E-3, -183.45940E-3, 55.25737E-3, -5.04421E-3, 1.44561E-3, -72.22364E-3}, 184.6E-3,
+         454.1E-3, 0, {}},
+        {"Ir", 3.88E-10, 0, 0, 222.3E9, 549.7E9,
+         {617.61574E-3, 3.28767E-3, -283.1158E-3, 96.12464E-3, -10.56298E-3, 157.20734E-3, -109.4680E-3}, 324.4E-3,
+         614.9E-3, 0, {}},
+        {"Mg", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {112.6205E-3, 0.7200E-3, -58.3781E-3, 28.1313E-3, -3.9212E-3, 25.2325E-3, -23.8545E-3}, 29.3216E-3,
+         88.4906E-3, 0, {1}},
+        {"MgY", 3.21E-10, 3, 0, 18.0E9, 46.8E9,
+         {126.3042E-3, -21.6452E-3, -29.3248E-3, 11.4053E-3, -1.2093E-3, -20.2756E-3, -10.9327E-3}, 30.0000E-3,
+         83.0000E-3, 0, {}},
+        {"Ti", 2.95E-10, 3, 0, 44.6E9, 118.67E9,
+         {112.6205E-3, 0.7200E-3, -58.3781E-3, 28.1313E-3, -3.9212E-3, 25.2325E-3, -23.8545E-3}, 333.565E-3,
+         622.5063E-3, 0,
+         {427.0E-3, -165.9E-3, -221.9E-3, -26.29E-3, -13.42E-3, -53.09E-3, 119.1E-3, -14.25E-3, -17.77E-3}},
+        {"MgAl", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {265.564638E-3, -96.111676E-3, 11.786691E-3, -7.169259E-3, 1.507026E-3, -161.871180E-3, 0.742991E-3},
+         23.52941E-3, 83.83E-3, 0, {1}},
+        {"MgCa", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {310.853100E-3, -114.443600E-3, 16.958600E-3, -11.221200E-3, 2.570400E-3, -191.432500E-3, 2.798000E-3},
+         31.56863E-3, 100.39216E-3, 0, {1}},
+        {"MgCe", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {241.210583E-3, -72.302347E-3, -13.746107E-3, 4.705126E-3, 0.523234E-3, 0.523234E-3, -7.398787E-3},
+         1.17647E-3, 95.19608E-3, 0, {1}},
+        {"MgGd", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {285.763113E-3, -95.960841E-3, 0.671163E-3, -1.588154E-3, 0.848439E-3, -159.569265E-3, -1.518973E-3},
+         11.66667E-3, 102.64706E-3, 0, {1}},
+        {"MgLi", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {281.067152E-3, -103.882445E-3, 16.615828E-3, -9.525039E-3, 1.569110E-3, -166.140969E-3, 4.570589E-3},
+         49.01961E-3, 99.41176E-3, 0, {1}},
+        {"MgSi", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {237.616625E-3, -85.750320E-3, 10.157547E-3, -5.707935E-3, 1.062990E-3, -148.577990E-3, -1.072479E-3},
+         14.90196E-3, 66.86275E-3, 0, {1}},
+        {"MgSn", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {265.383735E-3, -96.234971E-3, 12.703843E-3, -6.926203E-3, 1.011138E-3, -168.851904E-3, -0.639299E-3},
+         12.25490E-3, 72.84314E-3, 0, {1}},
+        {"MgZn", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {239.664844E-3, -83.074958E-3, 4.673015E-3, -3.589689E-3, 1.080065E-3, -132.943686E-3, -1.228621E-3},
+         33.03922E-3, 85.09804E-3, 0, {1}},
+        {"MgZr", 3.19E-10, 3, 0, 18.0E9, 46.8E9,
+         {288.840762E-3, -104.774131E-3, 13.194615E-3, -8.018332E-3, 1.683542E-3, -168.501923E-3, 2.901528E-3},
+         40.00000E-3, 100.78431E-3, 0, {1}},
+        {"Ta", 3.3058E-10, 1, 0, 70.67E9, 189.24E9, {}, 0, 950.21E-3, 0, {}},
+        {"W", 3.1652E-10, 1, 0, 160.6E9, 411.1E9, {}, 0, 1850.E-3, 0, {}},
+        {"Cr", 2.8848E-10, 1, 0, 115.4E9, 279.3E9, {}, 0, 0, 0, {}},
+        {"Fe", 2.8665E-10, 1, 0, 87.2796E9, 225.1814E9, {}, 0, 744.14E-3, 0, {}},
+        {"Nb", 3.324E-10, 2, 2.859E-10, 39.64E9, 110.33E9, {}, 0, 676.775E-3, 676.775E-3, {}},
+        {"Mo", 3.16E-10, 1, 0, 119.29E9, 310.68E9, {}, 0, 1.44339, 1.44339, {}},
+        {"MoNbTi", 3.2373185E-10, 1, 0, 32.30E9, 90.70E9, {}, 0, 0.761408228239499, 0.761408228239499, {}},
+        {"TaNbTi", 3.3014008E-10, 1, 0, 29.55E9, 82.71E9, {}, 0, 0.549789143600001, 0.549789143600001, {}},
+        {"Nb_ML", 3.326E-10, 1, 0, 24.89E9, 71.52E9, {}, 0, 0.77774, 0.77774, {}},
+        {"Mo_ML", 3.173E-10, 1, 0, 117.80E9, 305.67E9, {}, 0, 1.2321, 1.2321, {}},
+        {"Ta_ML", 3.322E-10, 1, 0, 48.79E9, 134.60E9, {}, 0, 0.76308, 0.76308, {}},
+    };
+    for (const Row &r : rows) {
+        if (name != r.name) continue;
+        a = r.a;
+        if (r.brule == 0) b = a / std::sqrt(2.0);
+        else if (r.brule == 1) b = std::sqrt(3.0) * a / 2.0;
+        else if (r.brule == 2) b = r.b;
+        mu = r.mu;
+        young = r.young;
+        c0 = r.c[0], c1 = r.c[1], c2 = r.c[2], c3 = r.c[3], c4 = r.c[4], a1 = r.c[5], a3 = r.c[6];
+        if (r.isf != 0) isf = r.isf;
+        if (r.usf != 0) usf = r.usf;
+        if (r.An != 0) An = r.An;
+        // {1} marks the Mg pyramidal-II constants the Mg alloys reuse
+        const double *py = (r.pyr[0] == 1 && r.pyr[1] == 0) ? MG_PYR : r.pyr;
+        for (int q = 0; q < 9; ++q) pyr[q] = py[q];
+        return true;
     }
-    return true;
+    return false;
 }
 
 void Material::derive()
@@ -222,6 +300,16 @@ std::vector<double> core_coefficients(const Material &m, int flag)
 {
     const double mdb = m.mu_s * m.dslip * m.b;
     if (flag == 1) return {m.An / mdb};
+    if (flag == 2) {
+        // core_energy_extended (fft_fftw_slab.cpp:1281-1287, 1318): c/mu, density/(dslip b)
+        return {m.c0 / mdb, m.c1 / mdb, m.c2 / mdb, m.c3 / mdb, m.c4 / mdb, m.a1 / mdb, m.a3 / mdb};
+    }
+    if (flag == 4 || flag == 9) {
+        // core_energy_pyrII (:1468-1476): aa0..aa4, bb1..bb4 over (mu dslip b)
+        std::vector<double> c(9);
+        for (int q = 0; q < 9; ++q) c[q] = m.pyr[q] / mdb;
+        return c;
+    }
     if (flag == 3) {
         const double isf = m.isf / mdb;
         const double Cn = (m.usf - (isf / 2.0)) / mdb;

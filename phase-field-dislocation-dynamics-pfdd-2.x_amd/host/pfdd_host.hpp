@@ -15,6 +15,7 @@ struct Material {
     double a = 0, b = 0, mu = 0, young = 0;
     double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, a1 = 0, a3 = 0;
     double isf = 0, usf = 0, An = 0, Cn = 0;
+    double pyr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // aa0..aa4, bb1..bb4 (pyramidal II)
     double dslip = 0.8165;
     // derived (material.cpp:156-167)
     double C44 = 0, C12 = 0, C11 = 0, nu = 0, ll = 0, S11 = 0, S12 = 0, S44 = 0;
@@ -22,7 +23,7 @@ struct Material {
     double mu_s = 0, ll_s = 0, young_s = 0, xnu_s = 0;
     std::vector<double> A_mpea; // per-cell USFE of core flag 7 (J/m^2), [a][i][j][k]
 
-    bool preset(const std::string &name); // Cu, Ni, Pd
+    bool preset(const std::string &name); // the named presets of material.cpp
     void derive();
 };
 

@@ -217,3 +217,27 @@ def test_deck_text_and_binary_dumps(tmp_path):
     assert np.array_equal(last[:, 3], xi[0].real) and np.array_equal(last[:, 4], xi[0].imag)
     assert np.array_equal(last[:, 5], xi[1].real)
     assert np.array_equal(last[:, 0], np.repeat(np.arange(16.0), 256))
+
+
+@pytestmark_cpu
+@pytest.mark.parametrize("preset,flag,names", [
+    ("Cu", 2, [344.948e-3, -85.399e-3, -40.421e-3, 12.396e-3, -0.674e-3, -131.393e-3, -19.816e-3]),
+    ("Ti", 4, [427.0e-3, -165.9e-3, -221.9e-3, -26.29e-3, -13.42e-3, -53.09e-3, 119.1e-3, -14.25e-3, -17.77e-3]),
+    ("MgZn", 9, [216.6e-3, -82.84e-3, -116.8e-3, -5.873e-3, -11.27e-3, -51.94e-3, 22.04e-3, 9.44e-3, -0.2501e-3]),
+])
+def test_check_mode_core_models_from_presets(tmp_path, preset, flag, names):
+    """Material presets (material.cpp:176-835) feed core_energy_extended (flag 2)
+    and core_energy_pyrII (flag 4; 9 = its Fourier model on every cell) with
+    coefficients over (mu dslip b) as fft_fftw_slab.cpp:1281-1287, 1468-1476."""
+    text = deck_text(app="fcc").replace("core_energy 3", f"core_energy {flag}")
+    text = text.replace("material mu 48e9 young 128.64e9 a 0.3615e-9 isf 41e-3 usf 158e-3",
+                        f"material preset {preset}")
+    r = run_exe(text, tmp_path, check=True)
+    assert r.returncode == 0, r.stderr
+    got = parse_check(r.stdout)
+    mu_young = {"Cu": (54.5e9, 144.4e9, 3.64e-10), "Ti": (44.6e9, 118.67e9, 2.95e-10), "MgZn": (18.0e9, 46.8e9, 3.19e-10)}
+    mu, young, a = mu_young[preset]
+    mat = oracle.material(mu, young)
+    mod = oracle.voigt(mat["C44"], mat["C12"], mat["C11"])
+    mdb = mod["mu"] * P.DSLIP * a / np.sqrt(2.0)
+    assert np.allclose(got["coef"], np.array(names) / mdb, rtol=1e-14)
