@@ -39,6 +39,12 @@ int pfdd_host_slip(const char *app, const double *prim9, const double *scale3, i
 int pfdd_host_resolss(const char *app, const double *prim9, const double *scale3, const double *sigma9,
                       double dslip, double *tau, double *sigma_rot9);
 
+/* rotate_stress + resolSS_non_Schmid (fft_fftw_slab.cpp:242-274, 774-845),
+ * the non_Schmid command's angle_to_110 (deg) and weights w1, w2, w3. */
+int pfdd_host_resolss_non_schmid(const char *app, const double *prim9, const double *scale3,
+                                 const double *sigma9, double dslip, double angle, double w1, double w2, double w3,
+                                 double *tau);
+
 /* core_energy_* coefficient normalisation (fft_fftw_slab.cpp:1165-1178,
  * 1396-1406): flag 1 -> coef[0] = An'; flag 3 -> coef[0..1] = isf', Cn'.
  * Returns the number of coefficients in *ncoef. */

@@ -47,6 +47,8 @@ def lib():
             "orc_rotate_slip": (None, [c_int, _dp, _dp, _dp, _dp]),
             "orc_rotate_stress": (None, [_dp, _dp, c_double, _dp, _dp]),
             "orc_resolss": (None, [c_int, _dp, _dp, _dp, c_double, _dp]),
+            "orc_resolss_non_schmid": (None, [c_int, _dp, _dp, c_double, c_double, c_double, c_double,
+                                              c_double, _dp]),
             "orc_frec": (None, [c_int, c_int, c_int, _dp, _dp, _dp, _dp]),
             "orc_stiffness": (None, [c_double, c_double, _dp]),
             "orc_bmatrix": (None, [c_int, c_int, c_int, c_int, c_double, c_double, c_double,
@@ -139,6 +141,14 @@ def resolss(sigma_rot, xn, xb, dslip):
     xn = _f64(xn)
     tau = np.zeros(xn.shape[0])
     lib().orc_resolss(xn.shape[0], _p(_f64(sigma_rot).reshape(-1)), _p(xn), _p(_f64(xb)), dslip, _p(tau))
+    return tau
+
+
+def resolss_non_schmid(sigma_rot, xb, dslip, angle, w1, w2, w3):
+    xb = _f64(xb).reshape(-1, 3)
+    tau = np.zeros(xb.shape[0])
+    lib().orc_resolss_non_schmid(xb.shape[0], _p(xb), _p(_f64(sigma_rot).reshape(-1)), float(dslip),
+                                 float(angle), float(w1), float(w2), float(w3), _p(tau))
     return tau
 
 

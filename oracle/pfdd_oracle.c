@@ -619,6 +619,39 @@ double orc_core_energy(int nx, int ny, int nz, int ns, const double *xi, int fla
     return E_ref;
 }
 
+/* resolSS_non_Schmid, :774-845, for ns systems with Burgers vectors xb
+ * (rows of 3): MRSS plane n_is at (angle + 120 is) deg about y, n'_is at
+ * +60 deg; tau = sum_ij sigma_rot_ij S_ij / dslip with
+ * S_ij = n_j b_i + w1 n'_j b_i + w2 n_j (n x b)_i + w3 n'_j (n' x b)_i. */
+void orc_resolss_non_schmid(int ns, const double *xb, const double *sigma_rot, double dslip, double angle,
+                            double w1, double w2, double w3, double *tau)
+{
+    double theta_n = 0.0;
+    for (int is = 0; is < ns; is++) {
+        double xm[3], xp[3], nxb[3], npxb[3];
+        const double *b = xb + 3 * is;
+        xm[0] = -sin((theta_n + angle) * M_PI / 180.);
+        xm[1] = 0.0;
+        xm[2] = cos((theta_n + angle) * M_PI / 180.);
+        xp[0] = -sin((theta_n + angle + 60) * M_PI / 180.);
+        xp[1] = 0.0;
+        xp[2] = cos((theta_n + angle + 60) * M_PI / 180.);
+        theta_n += 120.;
+        nxb[0] = xm[1] * b[2] - xm[2] * b[1];
+        nxb[1] = -xm[0] * b[2] + xm[2] * b[0];
+        nxb[2] = xm[0] * b[1] - xm[1] * b[0];
+        npxb[0] = xp[1] * b[2] - xp[2] * b[1];
+        npxb[1] = -xp[0] * b[2] + xp[2] * b[0];
+        npxb[2] = xp[0] * b[1] - xp[1] * b[0];
+        tau[is] = 0.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double S = xm[j] * b[i] + w1 * xp[j] * b[i] + w2 * xm[j] * nxb[i] + w3 * xp[j] * npxb[i];
+                tau[is] = tau[is] + (sigma_rot[3 * i + j] * S) / dslip;
+            }
+    }
+}
+
 /* data_real_a = sum_b BB[r + a N + b N S] * data_fftw_b : internal_energy,
  * :1670-1695 (loop order isa, isb, grid). */
 void orc_internal_energy(int nx, int ny, int nz, int ns, const double *BB,

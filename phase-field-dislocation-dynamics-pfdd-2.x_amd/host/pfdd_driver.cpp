@@ -74,6 +74,9 @@ void FFT_HIP::init()
         d.amp = amp.data();
     } else if (core_flag == 8) {
         d.core_model = PFDD_CORE_SERIES;
+    } else if (core_flag == 6) {
+        // sine core with An' of the stress state (set by resolSS each state)
+        d.core_model = PFDD_CORE_PERFECT;
     } else if (core_flag == 2) {
         d.core_model = PFDD_CORE_EXTENDED;
     } else if (core_flag == 4) {
@@ -106,6 +109,29 @@ void FFT_HIP::resolSS_Schmid()
     tau.assign(slip_systems, 0.0);
     resolss(slip, sigma_rot, mat.dslip, tau.data());
     check(pfdd_gl_set_tau(h_, tau.data()));
+}
+
+void FFT_HIP::resolSS_non_Schmid()
+{
+    tau.assign(slip_systems, 0.0);
+    resolss_non_schmid(slip, sigma_rot, mat.dslip, angle_to_110, w1, w2, w3, tau.data());
+    printf("angle %f, w1 %f,w2 %f,w3 %f \n", angle_to_110, w1, w2, w3);
+    for (int is = 0; is < slip_systems; ++is) printf("nonSchmid: Resolved shear stresses tau[%d] = %f\n", is, tau[is]);
+    check(pfdd_gl_set_tau(h_, tau.data()));
+}
+
+void FFT_HIP::resolSS()
+{
+    if (non_schmid) resolSS_non_Schmid();
+    else resolSS_Schmid();
+    if (core_flag == 6) {
+        // An' depends on the stress state: refreshed once per state, it is
+        // constant over the state's GL iterations
+        const double An = usfe_angle_tau(mat, sigma_rot, angle_to_110);
+        printf("core flag=6, usfe(angle,tau): %f\n", An * mat.mu_s * mat.dslip * mat.b * 1.0E3);
+        std::vector<double> c(slip_systems, An);
+        check(pfdd_gl_set_core_coef(h_, 1, c.data()));
+    }
 }
 
 int FFT_HIP::iterate_block(int n, double tol)
@@ -339,6 +365,12 @@ void App::init_app(FFT_HIP &fft)
         fft.coef = core_coefficients(fft.mat, fft.core_flag);
         fft.ncoef = (int)fft.coef.size();
     }
+    if (fft.core_flag == 6) {
+        double sr[3][3];
+        pfdd_host::rotate_stress(fft.geo, fft.sigma, sr);
+        fft.coef = {usfe_angle_tau(fft.mat, sr, fft.angle_to_110)};
+        fft.ncoef = 1;
+    }
     if (fft.core_flag == 2 && fft.slip_systems % 3)
         deck_error("core_energy 2 couples the slip systems of each plane in threes");
     if (fft.core_flag == 7 && fft.mat.A_mpea.empty()) deck_error("core_energy 7 needs 'material ... mpea FILE'");
@@ -385,7 +417,7 @@ void SolveGL::iterate(App &app, FFT_HIP &fft, Output &out)
     printf("Beginning Time Evolution\n");
     for (int sstate = 0; sstate < app.stopsteps; ++sstate) {
         fft.rotate_stress();
-        fft.resolSS_Schmid();
+        fft.resolSS();
         int it = 0;
         while (it < max_iter) {
             const double need = std::ceil(app.nextoutput - app.time); // first iteration that must output
@@ -570,6 +602,14 @@ void Input::command(const std::vector<std::string> &w)
         else deck_error("Invalid diag_style " + w[1]);
     } else if (c == "temperature") {
         num(w, 1, "temperature");
+    } else if (c == "non_Schmid") {
+        // App::set_nonSchmid (app.cpp:446-460): angle_to_110 w1 w2 w3
+        if (w.size() != 5) deck_error("Illegal non_Schmid command");
+        fft.non_schmid = true;
+        fft.angle_to_110 = num(w, 1, "non_Schmid");
+        fft.w1 = num(w, 2, "non_Schmid");
+        fft.w2 = num(w, 3, "non_Schmid");
+        fft.w3 = num(w, 4, "non_Schmid");
     } else if (c == "scale") {
         // App::set_scale (app.cpp:383-444)
         for (int q = 0; q < 3; ++q) app.scale[q] = num(w, 1 + q, "scale");
@@ -666,7 +706,10 @@ void Input::print_setup()
                fft.slip.xn[3 * a + 2], fft.slip.xb[3 * a], fft.slip.xb[3 * a + 1], fft.slip.xb[3 * a + 2]);
     rotate_stress(fft.geo, fft.sigma, fft.sigma_rot);
     std::vector<double> tau(fft.slip_systems);
-    resolss(fft.slip, fft.sigma_rot, fft.mat.dslip, tau.data());
+    if (fft.non_schmid)
+        resolss_non_schmid(fft.slip, fft.sigma_rot, fft.mat.dslip, fft.angle_to_110, fft.w1, fft.w2, fft.w3, tau.data());
+    else
+        resolss(fft.slip, fft.sigma_rot, fft.mat.dslip, tau.data());
     printf("tau");
     for (double v : tau) printf(" %.17g", v);
     printf("\nstopsteps %d max_iter %d tol %.17g\n", app.stopsteps, solve.max_iter, solve.tol);

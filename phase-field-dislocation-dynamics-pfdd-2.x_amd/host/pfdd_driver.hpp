@@ -52,6 +52,12 @@ class FFT_HIP {
     void init();  // FFTW_Slab::init + allocate + setup: device handle, B(k)
     void rotate_stress();
     void resolSS_Schmid();
+    void resolSS_non_Schmid();
+    // App::resolSS (app.cpp:654-658) + the per-stress-state An' of core flag 6
+    // (core_energy_USFE_angle_tau, fft_fftw_slab.cpp:1500-1545)
+    void resolSS();
+    bool non_schmid = false;                      // non_Schmid command (app.cpp:446-460)
+    double angle_to_110 = 0, w1 = 0, w2 = 0, w3 = 0;
     // The per-iteration virtuals of SolveGL::iterate.  init_loop, gradient,
     // core_energy, prep_forward, internal_energy, prep_backward and
     // prepare_next_itr are fused into the GPU iteration that

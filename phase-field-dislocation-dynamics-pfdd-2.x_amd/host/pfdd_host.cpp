@@ -115,6 +115,11 @@ bool Material::preset(const std::string &name)
         // {1} marks the Mg pyramidal-II constants the Mg alloys reuse
         const double *py = (r.pyr[0] == 1 && r.pyr[1] == 0) ? MG_PYR : r.pyr;
         for (int q = 0; q < 9; ++q) pyr[q] = py[q];
+        // USFE(angle, tau) fits of core flag 6 (material.cpp:713-717, 766-770)
+        if (name == "Ta") a_slope = 51.59816407, a_b = -2.27173784, b_b = 0.9979452, c_slope = -4.16536, c_b = 1133.961364;
+        if (name == "Fe")
+            a_slope = 15.524609942496577, a_b = 0.004388435835439675, b_b = 0.9934680885709689,
+            c_slope = -0.9065590784654822, c_b = 944.0633238797747;
         return true;
     }
     return false;
@@ -294,6 +299,42 @@ void resolss(const SlipTable &t, const double sigma_rot[3][3], double dslip, dou
             for (int j = 0; j < 3; ++j) v = v + (sigma_rot[i][j] * t.xn[3 * is + j] * t.xb[3 * is + i]) / dslip;
         tau[is] = v;
     }
+}
+
+void resolss_non_schmid(const SlipTable &t, const double sigma_rot[3][3], double dslip, double angle, double w1,
+                        double w2, double w3, double *tau)
+{
+    // FFTW_Slab::resolSS_non_Schmid (fft_fftw_slab.cpp:774-845; R. Groger et
+    // al., Acta Mater. 56 (2008)): per system the maximum-resolved-shear plane
+    // n (angle_to_110 + 120 deg per system about y), the plane n' at +60 deg,
+    // and tau = sigma_rot : [b n + w1 b n' + w2 (n x b) n + w3 (n' x b) n'] / dslip.
+    double theta = 0.0;
+    for (int is = 0; is < t.slip_systems; ++is, theta += 120.0) {
+        const double n[3] = {-std::sin((theta + angle) * M_PI / 180.), 0.0, std::cos((theta + angle) * M_PI / 180.)};
+        const double np[3] = {-std::sin((theta + angle + 60) * M_PI / 180.), 0.0,
+                              std::cos((theta + angle + 60) * M_PI / 180.)};
+        const double *b = &t.xb[3 * is];
+        const double nxb[3] = {n[1] * b[2] - n[2] * b[1], -n[0] * b[2] + n[2] * b[0], n[0] * b[1] - n[1] * b[0]};
+        const double npxb[3] = {np[1] * b[2] - np[2] * b[1], -np[0] * b[2] + np[2] * b[0], np[0] * b[1] - np[1] * b[0]};
+        double v = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double S = n[j] * b[i] + w1 * np[j] * b[i] + w2 * n[j] * nxb[i] + w3 * np[j] * npxb[i];
+                v = v + (sigma_rot[i][j] * S) / dslip;
+            }
+        tau[is] = v;
+    }
+}
+
+double usfe_angle_tau(const Material &m, const double sigma_rot[3][3], double angle)
+{
+    // core_energy_USFE_angle_tau (fft_fftw_slab.cpp:1500-1545): the unstable
+    // stacking-fault energy as a function of the MRSSP angle and the resolved
+    // shear sigma_rot[2][1] (mJ/m^2 fit), returned as An' = gsfe/(mu dslip b)
+    const double s21 = sigma_rot[2][1] * m.mu_s * 1.0E-9;
+    double g = (m.a_slope * s21 + m.a_b) * std::sin(m.b_b * angle * M_PI / 180.) + (m.c_slope * s21 + m.c_b);
+    g *= 1.0E-3;
+    return g / (m.mu_s * m.dslip * m.b);
 }
 
 std::vector<double> core_coefficients(const Material &m, int flag)

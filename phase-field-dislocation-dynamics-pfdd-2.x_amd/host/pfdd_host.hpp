@@ -16,6 +16,7 @@ struct Material {
     double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, a1 = 0, a3 = 0;
     double isf = 0, usf = 0, An = 0, Cn = 0;
     double pyr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // aa0..aa4, bb1..bb4 (pyramidal II)
+    double a_slope = 0, a_b = 0, b_b = 0, c_slope = 0, c_b = 0; // USFE(angle, tau) fit (flag 6)
     double dslip = 0.8165;
     // derived (material.cpp:156-167)
     double C44 = 0, C12 = 0, C11 = 0, nu = 0, ll = 0, S11 = 0, S12 = 0, S44 = 0;
@@ -63,6 +64,12 @@ void rotate_slip(SlipTable &t, const Geometry &g);
 void rotate_stress(const Geometry &g, const double sigma[3][3], double sigma_rot[3][3]);
 // FFTW_Slab::resolSS_Schmid (fft_fftw_slab.cpp:737-769).
 void resolss(const SlipTable &t, const double sigma_rot[3][3], double dslip, double *tau);
+// FFTW_Slab::resolSS_non_Schmid (fft_fftw_slab.cpp:774-845), the non_Schmid
+// command's angle_to_110 and weights w1..w3 (app.cpp:446-460).
+void resolss_non_schmid(const SlipTable &t, const double sigma_rot[3][3], double dslip, double angle, double w1,
+                        double w2, double w3, double *tau);
+// An' of core_energy_USFE_angle_tau (fft_fftw_slab.cpp:1500-1545).
+double usfe_angle_tau(const Material &m, const double sigma_rot[3][3], double angle);
 
 // Core-energy coefficients normalised as core_energy_* do (fft_fftw_slab.cpp:
 // 1165-1178, 1396-1406): flag 1 -> {An'}, flag 3 -> {isf', Cn'} with Cn' =

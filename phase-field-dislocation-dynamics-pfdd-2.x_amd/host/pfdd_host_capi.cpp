@@ -80,6 +80,24 @@ int pfdd_host_slip(const char *app, const double *prim9, const double *scale3, i
     return 0;
 }
 
+int pfdd_host_resolss_non_schmid(const char *app, const double *prim9, const double *scale3,
+                                 const double *sigma9, double dslip, double angle, double w1, double w2, double w3,
+                                 double *tau)
+{
+    if (!app || !sigma9 || !tau) return 1;
+    SlipTable t;
+    if (!slip_table(app, t)) return 1;
+    Geometry g;
+    geometry_from(prim9, scale3, g);
+    rotate_slip(t, g);
+    double s[3][3], sr[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s[r][c] = sigma9[3 * r + c];
+    rotate_stress(g, s, sr);
+    resolss_non_schmid(t, sr, dslip, angle, w1, w2, w3, tau);
+    return 0;
+}
+
 int pfdd_host_resolss(const char *app, const double *prim9, const double *scale3, const double *sigma9,
                       double dslip, double *tau, double *sigma_rot9)
 {

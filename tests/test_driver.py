@@ -241,3 +241,62 @@ def test_check_mode_core_models_from_presets(tmp_path, preset, flag, names):
     mod = oracle.voigt(mat["C44"], mat["C12"], mat["C11"])
     mdb = mod["mu"] * P.DSLIP * a / np.sqrt(2.0)
     assert np.allclose(got["coef"], np.array(names) / mdb, rtol=1e-14)
+
+
+@pytestmark_cpu
+def test_check_mode_non_schmid_and_usfe_angle_tau(tmp_path):
+    """non_Schmid (app.cpp:446-460) -> resolSS_non_Schmid (fft_fftw_slab.cpp:
+    774-845) against the oracle, and core_energy 6 (USFE(angle, tau),
+    :1500-1545) from the Ta preset's fit."""
+    s0 = "0.001 -0.002 0.0005 0.003 -0.004 0.002"
+    text = deck_text(app="fcc", s0=s0).replace("core_energy 3", "core_energy 6")
+    text = text.replace("material mu 48e9 young 128.64e9 a 0.3615e-9 isf 41e-3 usf 158e-3", "material preset Ta")
+    text = text.replace("mobility 0.01 1.0", "mobility 0.01 1.0\nnon_Schmid 25 0.3 -0.2 0.45")
+    r = run_exe(text, tmp_path, check=True)
+    assert r.returncode == 0, r.stderr
+    got = parse_check(r.stdout)
+    xb = np.array(got["xb"])
+    geo = P.geometry()
+    srot = oracle.rotate_stress(geo["prim"], geo["invprim"], geo["norvol"], sig_from6(six(s0)))
+    tau = oracle.resolss_non_schmid(srot, xb, P.DSLIP, 25.0, 0.3, -0.2, 0.45)
+    assert np.allclose(got["tau"], tau, rtol=1e-14, atol=1e-18)
+    mat = oracle.material(70.67e9, 189.24e9)
+    mu_s = oracle.voigt(mat["C44"], mat["C12"], mat["C11"])["mu"]
+    s21 = srot[2, 1] * mu_s * 1e-9
+    g = ((51.59816407 * s21 - 2.27173784) * np.sin(0.9979452 * 25 * np.pi / 180) + (-4.16536 * s21 + 1133.961364)) * 1e-3
+    b = np.sqrt(3.0) * 3.3058e-10 / 2.0
+    assert np.allclose(got["coef"], [g / (mu_s * P.DSLIP * b)], rtol=1e-13)
+
+
+@pytest.mark.gpu
+def test_deck_non_schmid_usfe_run_matches_oracle(tmp_path):
+    print("case: driver deck 16^3 x 3, non_Schmid + core_energy 6, ramp, 12 states", flush=True)
+    s0, ds = "0 0 0 0.001 0.002 0.0015", "0 0 0 0.0001 0.0001 0.0002"
+    text = deck_text(max_iter=2, tol=0, run=12, s0=s0, ds=ds, noise=1e-2, extra="write_xi xi_final.bin\n")
+    text = text.replace("core_energy 3", "core_energy 6")
+    text = text.replace("material mu 48e9 young 128.64e9 a 0.3615e-9 isf 41e-3 usf 158e-3", "material preset Fe")
+    text = text.replace("mobility 0.01 1.0", "mobility 0.01 1.0\nnon_Schmid 15 0.2 0.1 -0.3")
+    r = run_exe(text, tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    xi = np.fromfile(tmp_path / "xi_final.bin", dtype=np.float64).view(np.complex128).reshape(3, 16, 16, 16)
+    # oracle: SolveGL::iterate with resolSS_non_Schmid and the per-state An'
+    mat = oracle.material(87.2796e9, 225.1814e9)
+    mod = oracle.voigt(mat["C44"], mat["C12"], mat["C11"])
+    mat.update(dslip=P.DSLIP, b=np.sqrt(3.0) * 2.8665e-10 / 2.0, mu_s=mod["mu"], xnu=mod["xnu"])
+    case = P.make_case(shape=(16, 16, 16), noise=1e-2, loops=[(0, 2, 8, 8, 8, 4)], sigma_mu=0.0, core="perfect")
+    case["mat"] = mat
+    BB, _, _ = P.oracle_B(case)
+    st = oracle.GLState(case["xi0"], BB, case["cddt"], 1, np.zeros(1))
+    sigma, dsig, geo = sig_from6(six(s0)), sig_from6(six(ds)), case["geo"]
+    for _ in range(12):
+        srot = oracle.rotate_stress(geo["prim"], geo["invprim"], geo["norvol"], sigma)
+        tau = oracle.resolss_non_schmid(srot, case["xb"], P.DSLIP, 15.0, 0.2, 0.1, -0.3)
+        s21 = srot[2, 1] * mod["mu"] * 1e-9
+        g = ((15.524609942496577 * s21 + 0.004388435835439675) * np.sin(0.9934680885709689 * 15 * np.pi / 180)
+             + (-0.9065590784654822 * s21 + 944.0633238797747)) * 1e-3
+        st.core_par = np.array([g / (mod["mu"] * P.DSLIP * mat["b"])])
+        for _ in range(2):
+            st.step(tau)
+        sigma = sigma + dsig
+    err = np.max(np.abs(xi - st.xi)) / np.max(np.abs(st.xi))
+    assert err < 1e-9, err

@@ -353,3 +353,25 @@ def test_ranmar_published_test_values():
         r.uniform()
     got = [r.uniform() * 4096.0 * 4096.0 for _ in range(6)]
     assert got == [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0]
+
+
+def test_resolss_non_schmid_closed_forms():
+    """resolSS_non_Schmid (fft_fftw_slab.cpp:774-845) against the projection
+    tensor written as outer products, S = b (x) n + w1 b (x) n' + w2 (n x b) (x) n
+    + w3 (n' x b) (x) n', with n, n' the MRSS planes at angle + 120 k and +60 deg."""
+    rng = np.random.default_rng(17)
+    xb = rng.normal(size=(3, 3))
+    srot = rng.normal(size=(3, 3))
+    for angle, w in ((0.0, (0.0, 0.0, 0.0)), (27.5, (0.3, -0.2, 0.45))):
+        tau = oracle.resolss_non_schmid(srot, xb, P.DSLIP, angle, *w)
+        for k in range(3):
+            th = np.radians(angle + 120.0 * k)
+            n = np.array([-np.sin(th), 0.0, np.cos(th)])
+            npr = np.array([-np.sin(th + np.pi / 3), 0.0, np.cos(th + np.pi / 3)])
+            b = xb[k]
+            S = (np.outer(b, n) + w[0] * np.outer(b, npr) + w[1] * np.outer(np.cross(n, b), n)
+                 + w[2] * np.outer(np.cross(npr, b), npr))
+            assert abs(tau[k] - np.sum(srot * S) / P.DSLIP) < 1e-13 * (1 + abs(tau[k]))
+    # w = 0, angle 0: system 0 is the Schmid factor on the plane z
+    tau0 = oracle.resolss_non_schmid(srot, xb, P.DSLIP, 0.0, 0, 0, 0)[0]
+    assert abs(tau0 - xb[0] @ srot[:, 2] / P.DSLIP) < 1e-14
