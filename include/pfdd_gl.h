@@ -53,6 +53,9 @@ enum pfdd_gl_core {
                               each plane (ns % 3 == 0): coef[0..6] = c0..c4, a1, a3 over
                               (mu dslip b)                                                  */
     PFDD_CORE_SINE = 3,    /* flag 3, :1360-1418: coef[0] = isf', coef[1] = Cn'             */
+    PFDD_CORE_BCC = 5,     /* flag 5, :1071-1125 with gradient() :869-1065: coef[0] = usf',
+                              scaled per element by a quadratic in the dislocation
+                              character angle from the gradient of Re xi (no amp[])       */
     PFDD_CORE_PYRII = 4,   /* flag 4, :1429-1493 as executed: Fourier density
                               aa0 + sum_m aa_m cos(2 pi m x) + bb_m sin(2 pi m x), m <= 4,
                               coef[0..8] = aa0..aa4, bb1..bb4 over (mu dslip b); its force

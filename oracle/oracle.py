@@ -47,6 +47,7 @@ def lib():
             "orc_rotate_slip": (None, [c_int, _dp, _dp, _dp, _dp]),
             "orc_rotate_stress": (None, [_dp, _dp, c_double, _dp, _dp]),
             "orc_resolss": (None, [c_int, _dp, _dp, _dp, c_double, _dp]),
+            "orc_theta": (None, [c_int, c_int, c_int, c_int, _dp, _dp, _dp, _dp, _dp]),
             "orc_resolss_non_schmid": (None, [c_int, _dp, _dp, c_double, c_double, c_double, c_double,
                                               c_double, _dp]),
             "orc_frec": (None, [c_int, c_int, c_int, _dp, _dp, _dp, _dp]),
@@ -152,6 +153,15 @@ def resolss_non_schmid(sigma_rot, xb, dslip, angle, w1, w2, w3):
     return tau
 
 
+def theta(xi, four, xn, xb):
+    """Dislocation character angle of gradient() (quirk Q6 included)."""
+    S, nx, ny, nz = xi.shape
+    out = np.zeros((S, nx, ny, nz))
+    lib().orc_theta(nx, ny, nz, S, _p(np.ascontiguousarray(xi)), _p(_f64(four).reshape(-1)),
+                    _p(_f64(xn).reshape(-1)), _p(_f64(xb).reshape(-1)), _p(out))
+    return out
+
+
 def frec(nx, ny, nz, four):
     fx, fy, fz = (np.zeros((nx, ny, nz)) for _ in range(3))
     lib().orc_frec(nx, ny, nz, _p(_f64(four).reshape(-1)), _p(fx), _p(fy), _p(fz))
@@ -219,7 +229,7 @@ def core_energy(xi, flag, par, nm=0, amp=None):
 class GLState:
     """One reference GL run on a single rank (SolveGL::iterate, solve_GL.cpp:70-158)."""
 
-    def __init__(self, xi, BB, cddt, core_flag, core_par, nm=0, amp=None, xo=None):
+    def __init__(self, xi, BB, cddt, core_flag, core_par, nm=0, amp=None, xo=None, four=None, xn=None, xb=None):
         self.xi = np.ascontiguousarray(xi, dtype=np.complex128).copy()
         self.S, self.nx, self.ny, self.nz = self.xi.shape
         self.BB = np.ascontiguousarray(BB, dtype=np.float64)
@@ -231,6 +241,7 @@ class GLState:
         self.amp = _f64(amp) if amp is not None else None
         self.xo = _f64(xo) if xo is not None else None
         self.work = np.zeros(2 * 2 * self.S * self.nx * self.ny * self.nz)
+        self.four, self.xn, self.xb = four, xn, xb  # core flag 5 (theta)
 
     @property
     def data_fftw(self):
@@ -241,6 +252,9 @@ class GLState:
     def step(self, tau) -> dict:
         out = np.zeros(8)
         tau = _f64(tau)
+        if self.core_flag == 5:
+            # gradient() runs on the field of this iteration before core_energy
+            self.amp = _f64(theta(self.xi, self.four, self.xn, self.xb))
         lib().orc_gl_step(self.nx, self.ny, self.nz, self.S, _p(self.xi), _p(self.xo), _p(self.BB),
                           self.cddt, _p(tau), self.core_flag, _p(self.core_par), self.nm,
                           _p(self.amp), _p(self.work), _p(out))

@@ -524,6 +524,16 @@ double orc_core_energy(int nx, int ny, int nz, int ns, const double *xi, int fla
                 ed = A * (sin(M_PI * x) * sin(M_PI * x));
                 E_ref += ed / nx;
                 d = A * M_PI * sin(2.0 * M_PI * x);
+            } else if (flag == 5) {
+                /* core_energy_bcc_perfect, :1071-1125: usf' scaled by a
+                 * quadratic in the character angle theta (passed in amp[]) */
+                double th = amp[index], cof;
+                if (th < 0.0) cof = 1.0;
+                else if (th < 1.23) cof = 1.1603 * th * th - 2.0431 * th + 1;
+                else cof = 0.5473 * th * th - 2.0035 * th + 1.8923;
+                ed = par[0] * cof * (sin(M_PI * x) * sin(M_PI * x));
+                E_ref += ed / nx;
+                d = par[0] * cof * M_PI * sin(2.0 * M_PI * x);
             } else if (flag == 4 || flag == 9) {
                 /* core_energy_pyrII, :1429-1493: Fourier series
                  * par = aa0..aa4, bb1..bb4 (already / (mu dslip b)).  Flag 4
@@ -650,6 +660,47 @@ void orc_resolss_non_schmid(int ns, const double *xb, const double *sigma_rot, d
                 tau[is] = tau[is] + (sigma_rot[3 * i + j] * S) / dslip;
             }
     }
+}
+
+/* gradient(), :869-1065, one rank: periodic central differences of Re xi,
+ * transformed to the global axes with the reference's in-place order (the
+ * y and z components use the already transformed x (and y) component, quirk
+ * Q6), tang = grad x n, theta = acos(tang . b) flipped to pi - theta when
+ * (b x tang) . n < 0, and -1 where |tang| <= 0.1.  four row-major, xn/xb
+ * ns rows of 3. */
+void orc_theta(int nx, int ny, int nz, int ns, const double *xi, const double *four, const double *xn,
+               const double *xb, double *theta)
+{
+    long N = (long)nx * ny * nz;
+    for (int isa = 0; isa < ns; isa++)
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                for (int k = 0; k < nz; k++) {
+                    long index = (long)i * ny * nz + (long)j * nz + k + isa * N;
+                    long base = isa * N;
+#define RE(ii, jj, kk) xi[2 * (base + (long)(ii) * ny * nz + (long)(jj) * nz + (kk))]
+                    double gx = (RE((i + 1) % nx, j, k) - RE((i - 1 + nx) % nx, j, k)) / 2;
+                    double gy = (RE(i, (j + 1) % ny, k) - RE(i, (j - 1 + ny) % ny, k)) / 2;
+                    double gz = (RE(i, j, (k + 1) % nz) - RE(i, j, (k - 1 + nz) % nz)) / 2;
+#undef RE
+                    gx = gx * four[0] + gy * four[3] + gz * four[6];
+                    gy = gx * four[1] + gy * four[4] + gz * four[7];
+                    gz = gx * four[2] + gy * four[5] + gz * four[8];
+                    const double *n = xn + 3 * isa, *b = xb + 3 * isa;
+                    double t0 = gy * n[2] - gz * n[1], t1 = gz * n[0] - gx * n[2], t2 = gx * n[1] - gy * n[0];
+                    double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+                    if (mag <= 0.1) {
+                        theta[index] = -1;
+                    } else {
+                        t0 /= mag;
+                        t1 /= mag;
+                        t2 /= mag;
+                        theta[index] = acos(t0 * b[0] + t1 * b[1] + t2 * b[2]);
+                        double dir = (b[1] * t2 - b[2] * t1) * n[0] + (b[2] * t0 - b[0] * t2) * n[1] +
+                                     (b[0] * t1 - b[1] * t0) * n[2];
+                        if (dir < 0.0) theta[index] = acos(-1) - theta[index];
+                    }
+                }
 }
 
 /* data_real_a = sum_b BB[r + a N + b N S] * data_fftw_b : internal_energy,

@@ -815,6 +815,57 @@ __global__ void __launch_bounds__(NTHR)
     }
 }
 
+// ------------------------------------- character angle (core flag 5)
+// gradient() (fft_fftw_slab.cpp:869-1065) and the angle factor of
+// core_energy_bcc_perfect (:1071-1125) for one system (blockIdx.y) of the
+// field xo: periodic central differences of Re xi, transformed to the global
+// axes in the reference's in-place order (the y and z components reuse the
+// already transformed x (and y) component, quirk Q6), tang = grad x n,
+// theta = acos(tang . b), flipped to pi - theta when (b x tang) . n < 0, and
+// "no line" (factor 1) where |tang| <= 0.1.  Writes the factor cof(theta)
+// that scales usf' in the update.
+struct ThetaParams {
+    double four[9];
+    double xn[MAXS * 3], xb[MAXS * 3];
+    int nx, ny, nz, S;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(NTHR)
+    k_theta_cof(const typename cvec<R>::type *__restrict__ xo, R *__restrict__ cof, ThetaParams T, long long N,
+                const int *__restrict__ done)
+{
+    if (*done) return;
+    const long long r = (long long)blockIdx.x * NTHR + threadIdx.x;
+    if (r >= N) return;
+    const int a = blockIdx.y, as = a % T.S;
+    const int k = (int)(r % T.nz), j = (int)((r / T.nz) % T.ny), i = (int)(r / ((long long)T.nz * T.ny));
+    const typename cvec<R>::type *x = xo + (long long)a * N;
+    auto re = [&](int ii, int jj, int kk) {
+        return (double)x[((long long)ii * T.ny + jj) * T.nz + kk].x;
+    };
+    double gx = (re(i + 1 == T.nx ? 0 : i + 1, j, k) - re(i == 0 ? T.nx - 1 : i - 1, j, k)) / 2;
+    double gy = (re(i, j + 1 == T.ny ? 0 : j + 1, k) - re(i, j == 0 ? T.ny - 1 : j - 1, k)) / 2;
+    double gz = (re(i, j, k + 1 == T.nz ? 0 : k + 1) - re(i, j, k == 0 ? T.nz - 1 : k - 1)) / 2;
+    gx = gx * T.four[0] + gy * T.four[3] + gz * T.four[6];
+    gy = gx * T.four[1] + gy * T.four[4] + gz * T.four[7];
+    gz = gx * T.four[2] + gy * T.four[5] + gz * T.four[8];
+    const double *n = T.xn + 3 * as, *b = T.xb + 3 * as;
+    double t0 = gy * n[2] - gz * n[1], t1 = gz * n[0] - gx * n[2], t2 = gx * n[1] - gy * n[0];
+    const double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    double c = 1.0;
+    if (mag > 0.1) {
+        t0 /= mag;
+        t1 /= mag;
+        t2 /= mag;
+        double th = acos(t0 * b[0] + t1 * b[1] + t2 * b[2]);
+        const double dir = (b[1] * t2 - b[2] * t1) * n[0] + (b[2] * t0 - b[0] * t2) * n[1] + (b[0] * t1 - b[1] * t0) * n[2];
+        if (dir < 0.0) th = acos(-1.0) - th;
+        c = th < 1.23 ? 1.1603 * th * th - 2.0431 * th + 1 : 0.5473 * th * th - 2.0035 * th + 1.8923;
+    }
+    cof[(long long)a * N + r] = (R)c;
+}
+
 // ------------------------------------------ adjoint of pass 5 (reverse mode)
 // One GL iteration (k_update) maps xi to
 //   xi' = xi - cddt U (g/N - tau + dE(Re xi))      (Re part; Im: xi' = xi - cddt U g/N)
@@ -1391,6 +1442,7 @@ struct pfdd_gl {
     int nx = 0, ny = 0, nz = 0, S = 0, prec = PFDD_FP64;
     int M = 1, SM = 0;      // members of a batch, systems over all members (S*M)
     double *taud = nullptr; // device tau[SM] of a batch
+    void *cof = nullptr;    // core flag 5: per-element angle factor (R), refreshed every iteration
     unsigned version = 0;   // bumped when kernel arguments change (tau, core coefficients)
     long long N = 0;
     size_t esz = 16; // bytes per complex element
@@ -1410,6 +1462,7 @@ struct pfdd_gl {
     int nb3 = 0, nb5 = 0, nb5max = 0; // blocks of pass 3 / pass 5 (current tile, smallest tile)
     int tq[4] = {1, 1, 1, 1};          // tile shape of fwd z, fwd y, inv y, inv z + update
     int xpipe = 0;                     // x pass: 0 one tile per block, 1 persistent pipelined
+    ThetaParams TP{};                  // core flag 5
     int ncu = 0;                       // compute units of the device
     double *stage2 = nullptr; // NSLOT * NB2
     unsigned *ticket = nullptr;
@@ -1528,6 +1581,12 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     struct Disarm {
         ~Disarm() { t_ev_start = t_ev_stop = nullptr; }
     } disarm;
+    if (h->cof) {
+        // core flag 5: gradient() + the angle factor on the old field
+        hipLaunchKernelGGL((k_theta_cof<R>), dim3((unsigned)((h->N + NTHR - 1) / NTHR), h->SM), dim3(NTHR), 0, st,
+                           (const typename cvec<R>::type *)xo, (R *)h->cof, h->TP, h->N, h->done);
+        HIPCHK(hipGetLastError());
+    }
     arm(0);
     HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, nullptr,
                                   h->tq[0])));
@@ -1545,8 +1604,9 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     HIPCHK((launch_pass<R, true>(h->ny, T2, h->T1, m.p2o, m.p2i, m.n2, h->tw[1], h->done, st, nullptr,
                                  h->tq[2])));
     arm(4);
-    HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->amp, m.p5i, m.p5o, m.n5,
-                            (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done, st));
+    HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->cof ? h->cof : h->amp, m.p5i,
+                            m.p5o, m.n5, (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done,
+                            st));
     arm(5);
     // E_core as the reference accumulates it (sum / nx); core_energy_extended
     // and core_energy_pyrII do not accumulate it (fft_fftw_slab.cpp:1320,1483)
@@ -1734,7 +1794,7 @@ void destroy_all(pfdd_gl *h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->taud, h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
+    void *ptrs[] = {h->cof, h->taud, h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
                     h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket, h->dw[0], h->dw[1],
                     h->deps, h->dsig, h->tape.lam[0], h->tape.lam[1], h->tape.gpart, h->tape.gacc};
     for (void *p : ptrs)
@@ -1774,6 +1834,12 @@ int set_coef(pfdd_gl *h, int model, int ncoef, const double *coef, bool has_amp)
         if (!has_amp) return fail(PFDD_ERR_ARG, "core model 7 needs the per-cell amplitude amp[]");
         P.ncoef = 1;
         for (int a = 0; a < h->S; ++a) P.coef[a * MAXM] = 1.0;
+        break;
+    case PFDD_CORE_BCC:
+        if (!coef) return fail(PFDD_ERR_ARG, "core model 5 needs coef[0] = usf'");
+        if (has_amp) return fail(PFDD_ERR_UNSUPPORTED, "core model 5 takes no amp[]");
+        P.ncoef = 1;
+        for (int a = 0; a < h->S; ++a) P.coef[a * MAXM] = coef[0];
         break;
     case PFDD_CORE_EXTENDED:
         if (!coef) return fail(PFDD_ERR_ARG, "core model 2 needs coef[0..6] = c0..c4, a1, a3");
@@ -2242,6 +2308,18 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
                 return bail(fail(PFDD_ERR_HIP, "amp upload failed"));
         }
     }
+    if (d->core_model == PFDD_CORE_BCC) {
+        if ((rc = alloc(&h->cof, (size_t)h->SM * h->N * (h->prec == PFDD_FP64 ? 8 : 4)))) return bail(rc);
+        for (int q = 0; q < 9; ++q) h->TP.four[q] = d->four[q];
+        for (int q = 0; q < 3 * d->ns; ++q) {
+            h->TP.xn[q] = d->xn[q];
+            h->TP.xb[q] = d->xb[q];
+        }
+        h->TP.nx = h->nx;
+        h->TP.ny = h->ny;
+        h->TP.nz = h->nz;
+        h->TP.S = h->S;
+    }
     rc = h->prec == PFDD_FP64 ? build_B<double>(h) : build_B<float>(h);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(h->stream);
@@ -2352,6 +2430,7 @@ int pfdd_gl_set_core_coef(pfdd_gl *h, int ncoef, const double *coef)
 {
     int rc;
     if ((rc = check_handle(h))) return rc;
+    if (h->cof) return fail(PFDD_ERR_UNSUPPORTED, "core model 5 keeps its usf' (set at create)");
     ++h->version;
     return set_coef(h, PFDD_CORE_SERIES, ncoef, coef, h->amp != nullptr);
 }
@@ -2378,7 +2457,8 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
     if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
-    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_PYRII || h->P.model == PFDD_CORE_FOURIER)
+    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_PYRII || h->P.model == PFDD_CORE_FOURIER ||
+        h->P.model == PFDD_CORE_BCC)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode supports the sine-series core models (0, 1, 3, 7, 8)");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3]) : tile_lines_rt<float>(h->nz, h->tq[3]);
     if (((long long)h->nx * h->ny) % nl != 0)

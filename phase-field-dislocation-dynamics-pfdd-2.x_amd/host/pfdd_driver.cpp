@@ -77,6 +77,8 @@ void FFT_HIP::init()
     } else if (core_flag == 6) {
         // sine core with An' of the stress state (set by resolSS each state)
         d.core_model = PFDD_CORE_PERFECT;
+    } else if (core_flag == 5) {
+        d.core_model = PFDD_CORE_BCC;
     } else if (core_flag == 2) {
         d.core_model = PFDD_CORE_EXTENDED;
     } else if (core_flag == 4) {
@@ -361,7 +363,8 @@ void App::init_app(FFT_HIP &fft)
     fft.CD = CD >= 0 ? CD : fft.slip.CD;
     fft.timestep = timestep >= 0 ? timestep : fft.slip.timestep;
     fft.core_flag = core_flag >= 0 ? core_flag : fft.slip.default_core;
-    if (fft.core_flag == 1 || fft.core_flag == 2 || fft.core_flag == 3 || fft.core_flag == 4 || fft.core_flag == 9) {
+    if (fft.core_flag == 1 || fft.core_flag == 2 || fft.core_flag == 3 || fft.core_flag == 4 || fft.core_flag == 5 ||
+        fft.core_flag == 9) {
         fft.coef = core_coefficients(fft.mat, fft.core_flag);
         fft.ncoef = (int)fft.coef.size();
     }

@@ -341,6 +341,7 @@ std::vector<double> core_coefficients(const Material &m, int flag)
 {
     const double mdb = m.mu_s * m.dslip * m.b;
     if (flag == 1) return {m.An / mdb};
+    if (flag == 5) return {m.usf / mdb}; // core_energy_bcc_perfect (:1090-1094)
     if (flag == 2) {
         // core_energy_extended (fft_fftw_slab.cpp:1281-1287, 1318): c/mu, density/(dslip b)
         return {m.c0 / mdb, m.c1 / mdb, m.c2 / mdb, m.c3 / mdb, m.c4 / mdb, m.a1 / mdb, m.a3 / mdb};

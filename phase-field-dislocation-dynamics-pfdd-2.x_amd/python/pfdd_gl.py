@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfdd_gl.so")
 
 PFDD_FP64, PFDD_FP32 = 0, 1
 CORE_NONE, CORE_PERFECT, CORE_SINE, CORE_MPEA, CORE_SERIES = 0, 1, 3, 7, 8
-CORE_EXTENDED, CORE_PYRII, CORE_FOURIER = 2, 4, 9
+CORE_EXTENDED, CORE_PYRII, CORE_BCC, CORE_FOURIER = 2, 4, 5, 9
 MAX_SYSTEMS = 12
 MAX_COEF = 8
 FLAG_STORED_B = 1

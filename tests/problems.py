@@ -143,6 +143,11 @@ def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigm
             par[:, 2:] = 0.1 * base[0] * (2 * rng.random((S, ncoef - 2)) - 1)
         flag, nm = 8, ncoef
         gpu_core = dict(core_model=8, coef=par.reshape(-1), ncoef=ncoef)
+    elif core == "bcc":
+        mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
+        par = np.array([USF / mdb])
+        flag, nm = 5, 0
+        gpu_core = dict(core_model=5, coef=par, ncoef=0)
     elif core == "extended":
         # Cu gamma-surface constants of material.cpp (c0..c4, a1, a3, J/m^2)
         mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
@@ -178,7 +183,7 @@ def oracle_state(case, BB=None):
     if BB is None:
         BB, _, _ = oracle_B(case)
     return oracle.GLState(case["xi0"], BB, case["cddt"], case["core_flag"], case["core_par"],
-                          case["nm"], None, case["xo"])
+                          case["nm"], None, case["xo"], four=case["geo"]["four"], xn=case["xn"], xb=case["xb"])
 
 
 def gpu_solver(case, precision=0, stored_B=False):

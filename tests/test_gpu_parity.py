@@ -227,13 +227,18 @@ def test_full_size_c1_properties_fp64():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("core,systems,precision", [("extended", "basal3", 0), ("extended", "fcc12", 0),
-                                                    ("pyrII", "basal3", 0), ("fourier", "basal3", 0),
-                                                    ("extended", "basal3", 1), ("fourier", "basal3", 1)])
-def test_other_core_models_match_oracle(core, systems, precision):
-    print(f"case: core model {core}, 16^3 {systems}, precision {precision}, 20 steps", flush=True)
+@pytest.mark.parametrize("core,systems,precision,skew", [("extended", "basal3", 0, False),
+                                                         ("extended", "fcc12", 0, False),
+                                                         ("pyrII", "basal3", 0, False), ("fourier", "basal3", 0, False),
+                                                         ("extended", "basal3", 1, False),
+                                                         ("fourier", "basal3", 1, False),
+                                                         ("bcc", "basal3", 0, False), ("bcc", "basal3", 0, True),
+                                                         ("bcc", "basal3", 1, True)])
+def test_other_core_models_match_oracle(core, systems, precision, skew):
+    print(f"case: core model {core}, 16^3 {systems}, precision {precision}, skew {skew}, 20 steps", flush=True)
+    prim = ((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1)) if skew else ((1, 0, 0), (0, 1, 0), (0, 0, 1))
     case = P.make_case(shape=(16, 16, 16), systems=systems, core=core, noise=2e-2, sigma_mu=0.004,
-                       loops=[(0, 2, 8, 8, 8, 4)])
+                       loops=[(0, 2, 8, 8, 8, 4)], prim_vectors=prim)
     st = P.oracle_state(case)
     s = P.gpu_solver(case, precision=precision)
     for _ in range(20):

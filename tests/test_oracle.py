@@ -375,3 +375,30 @@ def test_resolss_non_schmid_closed_forms():
     # w = 0, angle 0: system 0 is the Schmid factor on the plane z
     tau0 = oracle.resolss_non_schmid(srot, xb, P.DSLIP, 0.0, 0, 0, 0)[0]
     assert abs(tau0 - xb[0] @ srot[:, 2] / P.DSLIP) < 1e-14
+
+
+def test_theta_character_angle():
+    """gradient() (fft_fftw_slab.cpp:869-1065): edge segment -> pi/2, screw ->
+    0 or pi, no line -> -1, and the in-place transform to global axes (quirk
+    Q6) on a sheared reciprocal matrix, by hand."""
+    xi = np.zeros((1, 8, 8, 4), dtype=complex)
+    xi[0, :4] = 1.0  # slipped for i < 4 on every plane: lines along y at i = 3.5 and 7.5
+    n = np.array([[0.0, 0.0, 1.0]])
+    th = oracle.theta(xi, np.eye(3), n, np.array([[1.0, 0.0, 0.0]]))
+    assert np.allclose(th[0, [3, 4, 7, 0]], np.pi / 2)
+    assert np.all(th[0, [1, 2, 5, 6]] == -1.0)
+    ths = oracle.theta(xi, np.eye(3), n, np.array([[0.0, 1.0, 0.0]]))
+    assert np.all(np.isclose(ths[0, [3, 4]], 0.0) | np.isclose(ths[0, [3, 4]], np.pi))
+    # four = [[1, s, 0], [0, 1, 0], [0, 0, 1]]: at i = 3 the raw gradient is (-1/2, 0, 0);
+    # in place: gx' = gx, gy' = gx' s, gz' = 0, so tang = (gy', -gx', 0)
+    s_ = 0.5
+    four = np.array([[1.0, s_, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    b = np.array([1.0, 0.0, 0.0])
+    thq = oracle.theta(xi, four, n, b[None])
+    gx = -0.5
+    t = np.array([gx * s_, -gx, 0.0])
+    t /= np.linalg.norm(t)
+    want = np.arccos(t @ b)
+    if np.cross(b, t) @ n[0] < 0:
+        want = np.pi - want
+    assert np.allclose(thq[0, 3], want, rtol=1e-14)
