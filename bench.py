@@ -18,6 +18,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -106,6 +107,9 @@ def load_pmc_traffic(tag, kernel):
 KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update": "inv_z_update"}
 
 
+PROBE_STEPS = 3
+
+
 def pmc_probe(size):
     """Workload for the rocprofv3 --pmc runs: a plain element-wise copy of known
     size (torch's element-wise multiply by 1: ordinary loads and stores, the
@@ -124,7 +128,7 @@ def pmc_probe(size):
     prob = make_workload(size)
     for prec in (pfdd_gl.PFDD_FP64, pfdd_gl.PFDD_FP32):
         s = solver_for(prob, prec)
-        s.step(3)
+        s.step(PROBE_STEPS)
         s.close()
         print(f"pmc-probe: precision {prec} done", flush=True)
     return 0
@@ -151,13 +155,17 @@ def pmc_summarize(fetch_dir, write_dir, size, out_dir):
     fetch, write = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
     nbytes = float(1 << 30)
 
-    def per_kernel(rows, match):
-        v = [val for (d, name), val in rows.items() if match(name)]
-        return sum(v) / len(v) if v else None
+    def per_kernel(rows, match, last=None):
+        # the probe's own GL iterations are the last dispatches of each kernel
+        # role (the autotuner's candidate launches come before them)
+        v = sorted(((int(d), name, val) for (d, name), val in rows.items() if match(name)))
+        if last:
+            v = v[-last:]
+        return (sum(x[2] for x in v) / len(v), v[-1][1]) if v else (None, None)
 
     is_copy = lambda n: "elementwise" in n and "Mul" in n
-    copy_f = per_kernel(fetch, is_copy)
-    copy_w = per_kernel(write, is_copy)
+    copy_f, _ = per_kernel(fetch, is_copy)
+    copy_w, _ = per_kernel(write, is_copy)
     copy_name = next((name for (d, name) in fetch if is_copy(name)), None)
     sf = nbytes / (copy_f * 1024.0) if copy_f else 1.0
     sw = nbytes / (copy_w * 1024.0) if copy_w else 1.0
@@ -173,10 +181,13 @@ def pmc_summarize(fetch_dir, write_dir, size, out_dir):
                                     "fetch_scale": sf, "write_scale": sw}, "kernels": {}}
         for sub, role in KERNEL_ROLES.items():
             m = lambda n, sub=sub: sub in n and f"<{tname}," in n
-            f, w = per_kernel(fetch, m), per_kernel(write, m)
+            (f, kname), (w, _) = per_kernel(fetch, m, PROBE_STEPS), per_kernel(write, m, PROBE_STEPS)
             if f is None or w is None:
                 continue
-            res["kernels"][role] = {"fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
+            km = re.search(r"(k_\w+<[^>]*>)", kname)
+            res["kernels"][role] = {"kernel": km.group(1) if km else kname,
+                                    "launches": PROBE_STEPS,
+                                    "fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
                                     "hbm_bytes_per_launch": f * 1024.0 * sf + w * 1024.0 * sw,
                                     "algorithmic_bytes": algo[role]}
         out[tag] = res
@@ -475,11 +486,17 @@ def main():
     for prec, tag in ((pfdd_gl.PFDD_FP64, "fp64"), (pfdd_gl.PFDD_FP32, "fp32")):
         s = solver_for(prob, prec)
         dt, r = time_steps(s, args.steps, args.warmup)
-        xi_end = s.get_xi() if tag else None
+        t0 = time.perf_counter()
+        xi_end = s.get_xi()  # device -> host copy of the field (FP64 host layout)
+        t_down = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.set_xi(xi_end)     # host -> device
+        t_up = time.perf_counter() - t0
         roof = roofline(s)
         roof["traffic"] = load_pmc_traffic(tag, roof["kernel"])
         roof["tiles"] = s.autotune()  # cached choice of the create-time tuning
         out[tag] = {"sec": dt, "steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                    "host_copy_ms": {"upload": 1e3 * t_up, "download": 1e3 * t_down},
                     "roofline": roof, "e_total": r["e_total"], "xi_end": xi_end}
         s.close()
     # FP32 error against the FP64 GPU run of the same length
@@ -522,6 +539,12 @@ def main():
             "fp32": {"bytes": f32["roofline"]["step_bytes"],
                      "frac": f32["roofline"]["step_bytes"] / (f32["ms_per_step"] * 1e-3) / PEAK_HBM}},
         "per_kernel": {"fp64": f64["roofline"]["per_kernel"], "fp32": f32["roofline"]["per_kernel"]},
+        # the same K steps with the field handed over in host memory (one
+        # upload before, one download after: the C ABI's set_xi / get_xi)
+        "pcie_inclusive": {
+            tag: {"host_copy_ms": o["host_copy_ms"],
+                  "steps_per_s": args.steps / (o["sec"] + 1e-3 * sum(o["host_copy_ms"].values()))}
+            for tag, o in (("fp64", f64), ("fp32", f32))},
         "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update", "x_contract (1 = pipelined)"],
                             "fp64": f64["roofline"]["tiles"], "fp32": f32["roofline"]["tiles"]},
     }

@@ -217,7 +217,8 @@ int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, dou
  * over `nsteps` GL iterations on this handle's own buffers, and keeps the
  * fastest per kernel (choice[5], may be NULL).  The field and step counters
  * are restored; the choice is cached for the process per grid, system count,
- * batch, precision and options.  Results (xi) do not depend on the choice. */
+ * batch, precision and options.  Results (xi) do not depend on the choice.
+ * The update of core models 2, 4 and 9 always uses tile shape 0. */
 int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5]);
 /* Sets the choices explicitly (same meaning as pfdd_gl_autotune's output). */
 int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5]);

@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(NTHR)
 // does (it reads data_fftw = xi before the update).  Masked cells keep their
 // value; their xinorm term uses the previous unmasked cell's old value in
 // (a, i, j, k) order (prev[] precomputed on the host, -1 = the initial 0).
-template <typename R, int N, int TQ>
+template <typename R, int N, int TQ, bool GEN>
 __global__ void __launch_bounds__(NTHR)
     k_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ xo,
              typename cvec<R>::type *__restrict__ xn, const uint8_t *__restrict__ mask,
@@ -731,7 +731,7 @@ __global__ void __launch_bounds__(NTHR)
         x = xold[t];
         // core energy density and its derivative at Re xi of the old field
         R dens = (R)0, dE = (R)0;
-        if (P.model == PFDD_CORE_EXTENDED) {
+        if (GEN && P.model == PFDD_CORE_EXTENDED) {
             // the other two systems of the plane at the same cell
             const int j = lsub[l];
             const long long b0 = idx - (long long)j * P.ncell;
@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(NTHR)
             R sp, cp;
             if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
             else sincospif((float)x.x, (float *)&sp, (float *)&cp);
-            if (P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER) {
+            if (GEN && (P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER)) {
                 // a0 + sum_m a_m cos(2 pi m x) + b_m sin(2 pi m x), m <= 4
                 const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
                 R sm = s1, cm = c1;
@@ -1255,12 +1255,24 @@ hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn
                          long long pstride, const int *done, hipStream_t st)
 {
     typedef typename cvec<R>::type V;
+    // the coupled and Fourier core models get their own instantiation (one
+    // tile shape) so that the common sine-series kernel carries none of
+    // their registers
+    const bool gen = P.model == PFDD_CORE_EXTENDED || P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER;
 #define PFDD_CASE(n_)                                                                                \
     if (n == n_) {                                                                                   \
+        if (gen) {                                                                                   \
+            constexpr int NL = tile_lines_q<R, n_, 0>();                                             \
+            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
+            PFDD_LAUNCH((k_update<R, n_, 0, true>), dim3(nb), dim3(NTHR), st, (const V *)T1,         \
+                        (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, P,  \
+                        (const V *)tw, partial, pstride, done);                                      \
+            return hipGetLastError();                                                                \
+        }                                                                                            \
         PFDD_TQ_DISPATCH(tq, {                                                                       \
             constexpr int NL = tile_lines_q<R, n_, TQ>();                                            \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
-            PFDD_LAUNCH((k_update<R, n_, TQ>), dim3(nb), dim3(NTHR), st, (const V *)T1,              \
+            PFDD_LAUNCH((k_update<R, n_, TQ, false>), dim3(nb), dim3(NTHR), st, (const V *)T1,       \
                                (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, \
                                P, (const V *)tw, partial, pstride, done);                            \
         })                                                                                           \
@@ -1632,9 +1644,14 @@ int step_once(pfdd_gl *h, double tol, hipEvent_t *ev)
 void set_tiles(pfdd_gl *h, const int tq[5])
 {
     for (int p = 0; p < 4; ++p) h->tq[p] = tq[p];
+    // the update of the coupled and Fourier core models has one tile shape,
+    // the smallest (fewest values per thread: no spills in FP32)
+    const int m = h->P.model;
+    if (m == PFDD_CORE_EXTENDED || m == PFDD_CORE_PYRII || m == PFDD_CORE_FOURIER) h->tq[3] = 0;
     h->xpipe = h->stored_B ? 0 : tq[4];
     const long long n5 = (long long)h->SM * h->nx * h->ny;
-    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, tq[3]) : tile_lines_rt<float>(h->nz, tq[3]);
+    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3])
+                                        : tile_lines_rt<float>(h->nz, h->tq[3]);
     h->nb5 = (int)((n5 + nl - 1) / nl);
     ++h->version; // captured graphs hold the old launch shapes
 }
@@ -2262,6 +2279,10 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
         h->P.taud = h->taud;
     }
     h->pstride = h->nb5max > h->nb3 ? h->nb5max : h->nb3;
+    {
+        const int dflt[5] = {1, 1, 1, 1, 0};
+        set_tiles(h, dflt); // nb5 of the model's update tile
+    }
     if ((rc = alloc((void **)&h->partial, sizeof(double) * NSLOT * h->pstride))) return bail(rc);
     if ((rc = alloc((void **)&h->res, sizeof(double) * NRES))) return bail(rc);
     if ((rc = alloc((void **)&h->stage2, sizeof(double) * NSLOT * NB2)) ||

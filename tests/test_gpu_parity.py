@@ -252,3 +252,26 @@ def test_other_core_models_match_oracle(core, systems, precision, skew):
     assert abs(r["e_core"] - ref["E_core"]) <= tol * max(1.0, abs(ref["E_core"])) * 10
     assert abs(r["e_core_ref"] - ref["E_core_ref"]) <= tol * max(1.0, abs(ref["E_core_ref"]))
     assert abs(r["e_total"] - ref["E_total"]) <= tol * max(1.0, abs(ref["E_total"])) * 10
+
+
+def test_gpu_matches_committed_fixtures():
+    """The GPU path against the committed FP64 fixtures (tests/golden), which
+    the oracle reproduces in the CPU suite: fields and scalars within 1e-9."""
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_fixtures as F
+    fx = np.load(os.path.join(here, "golden", "gl_fixtures.npz"))
+    for name, (kw, steps) in F.cases().items():
+        _say(f"case: committed fixture {name}, {steps} steps, FP64")
+        c = P.make_case(**kw)
+        g = P.gpu_solver(c)
+        try:
+            r = g.step(steps)
+            xi = g.get_xi()
+        finally:
+            g.close()
+        assert relerr(xi, fx[name + "/xi"]) < 1e-9, (name, relerr(xi, fx[name + "/xi"]))
+        sc = fx[name + "/scalars"]
+        for v, w in zip((r["xinorm"], r["xiave"], r["e_total"]), sc[1:]):
+            assert scal_rel(v, w) < 1e-9, (name, v, w)

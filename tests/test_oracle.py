@@ -402,3 +402,24 @@ def test_theta_character_angle():
     if np.cross(b, t) @ n[0] < 0:
         want = np.pi - want
     assert np.allclose(thq[0, 3], want, rtol=1e-14)
+
+
+def test_oracle_reproduces_committed_fixtures():
+    """tests/golden/gl_fixtures.npz (written by tests/golden/make_fixtures.py):
+    the seeded recipe still gives the same initial fields (checksums) and the
+    oracle still gives the committed fields and scalars."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_fixtures as F
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gl_fixtures.npz"))
+    for name, (kw, steps) in F.cases().items():
+        c = P.make_case(**kw)
+        assert np.allclose(F.checksum(c["xi0"]), fx[name + "/xi0_checksum"], rtol=1e-13, atol=0), name
+        _, xi, r = F.run_oracle(kw, steps)
+        ref = fx[name + "/xi"]
+        assert np.max(np.abs(xi - ref)) / np.max(np.abs(ref)) < 1e-12, name
+        sc = fx[name + "/scalars"]
+        assert int(sc[0]) == steps
+        for v, w in zip((r["xinorm"], r["xiave"], r["E_total"]), sc[1:]):
+            assert abs(v - w) <= 1e-12 * max(abs(w), 1.0), (name, v, w)
