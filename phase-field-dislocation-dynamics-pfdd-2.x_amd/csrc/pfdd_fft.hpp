@@ -46,6 +46,32 @@ template <bool INV, typename V> __device__ __forceinline__ V cmul_mi(V a)
 // interleaved (8-byte accesses); double2 values are kept as two planes of
 // doubles (re, im) so that every LDS access is 8 bytes wide: 16-byte
 // accesses to the same tile patterns measured heavy bank conflicts.
+// Streaming accesses of the field passes.  Every element is read once and
+// written once per pass, so the loads carry the nontemporal hint (measured on
+// MI355X at 128^3 x 12: FP32 step -17 %, FP64 -6 %).  Stores carry it for
+// 16-byte elements only: for 8-byte elements it slowed the FP32 line passes
+// by a quarter, for 16-byte elements it gained another 3 %.
+template <typename V> __device__ __forceinline__ V ld_stream(const V *p)
+{
+    typedef decltype(V{}.x) E;
+    typedef E E2 __attribute__((ext_vector_type(2)));
+    const E2 t = __builtin_nontemporal_load(reinterpret_cast<const E2 *>(p));
+    return mk<V>(t.x, t.y);
+}
+template <typename V> __device__ __forceinline__ void st_stream(V *p, V v)
+{
+    if constexpr (sizeof(V) >= 16) {
+        typedef decltype(V{}.x) E;
+        typedef E E2 __attribute__((ext_vector_type(2)));
+        E2 t;
+        t.x = v.x;
+        t.y = v.y;
+        __builtin_nontemporal_store(t, reinterpret_cast<E2 *>(p));
+    } else {
+        *p = v;
+    }
+}
+
 template <typename V> struct Tile;
 template <> struct Tile<float2> {
     float2 *p;
@@ -253,14 +279,14 @@ __device__ __forceinline__ void tile_load(const Tile<V> lds, const V *__restrict
             for (int e = tid; e < TOT; e += NTHR) {
                 const int l = e / N, p = e % N;
                 const long long b = base[l];
-                if (b >= 0) lds.st(p * LDP + l, src[b + p]);
+                if (b >= 0) lds.st(p * LDP + l, ld_stream(&src[b + p]));
             }
         } else {
 #pragma unroll 4
             for (int e = tid; e < TOT; e += NTHR) {
                 const int l = e % NL, p = e / NL;
                 const long long b = base[l];
-                if (b >= 0) lds.st(p * LDP + l, src[b + p * SP]);
+                if (b >= 0) lds.st(p * LDP + l, ld_stream(&src[b + p * SP]));
             }
         }
     } else {
@@ -270,7 +296,7 @@ __device__ __forceinline__ void tile_load(const Tile<V> lds, const V *__restrict
             for (int t = 0; t < PER; ++t) {
                 const int e = tid + t * NTHR;
                 const long long b = (TOT % NTHR == 0 || e < TOT) ? base[e / N] : -1;
-                if (b >= 0) r[t] = src[b + e % N];
+                if (b >= 0) r[t] = ld_stream(&src[b + e % N]);
             }
 #pragma unroll
             for (int t = 0; t < PER; ++t) {
@@ -282,7 +308,7 @@ __device__ __forceinline__ void tile_load(const Tile<V> lds, const V *__restrict
             for (int t = 0; t < PER; ++t) {
                 const int e = tid + t * NTHR;
                 const long long b = (TOT % NTHR == 0 || e < TOT) ? base[e % NL] : -1;
-                if (b >= 0) r[t] = src[b + (e / NL) * SP];
+                if (b >= 0) r[t] = ld_stream(&src[b + (e / NL) * SP]);
             }
 #pragma unroll
             for (int t = 0; t < PER; ++t) {
@@ -303,14 +329,14 @@ __device__ __forceinline__ void tile_store(V *__restrict__ dst, const Tile<V> ld
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e / N, p = e % N;
             const long long b = base[l];
-            if (b >= 0) dst[b + p] = lds.ld(p * LDP + l);
+            if (b >= 0) st_stream(&dst[b + p], lds.ld(p * LDP + l));
         }
     } else {
 #pragma unroll 4
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e % NL, p = e / NL;
             const long long b = base[l];
-            if (b >= 0) dst[b + p * SP] = lds.ld(p * LDP + l);
+            if (b >= 0) st_stream(&dst[b + p * SP], lds.ld(p * LDP + l));
         }
     }
 }

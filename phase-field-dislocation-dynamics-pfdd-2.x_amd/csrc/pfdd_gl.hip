@@ -251,7 +251,7 @@ __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__res
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e / N, p = e % N;
             const int a = l / NP, q = l % NP;
-            if (c0 + q < ncol) lds.st(p * LDP + l, T2[(long long)a * ntot + (c0 + q) * N + p]);
+            if (c0 + q < ncol) lds.st(p * LDP + l, ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]));
         }
     } else {
         V r[PER];
@@ -260,7 +260,7 @@ __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__res
             const int e = tid + t * NTHR;
             const int l = e / N, p = e % N;
             const int a = l / NP, q = l % NP;
-            if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = T2[(long long)a * ntot + (c0 + q) * N + p];
+            if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]);
         }
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) lds.st(p * LDP + l, T2[(long long)a * ntot + (c0 + q) * N + p]);
+        if (c0 + q < ncol) lds.st(p * LDP + l, ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]));
     }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
+        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
     }
     double v[1] = {el};
     block_sum<1, NTHR>(v, red, tid);
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) T2[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
+        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
     }
     block_sum<1, NTHR>(v, red, tid);
     if (tid == 0) part_el[blockIdx.x] = v[0];
@@ -589,7 +589,7 @@ __device__ __forceinline__ void cols_fetch(V *r, const V *__restrict__ T2, long 
         const int e = tid + t * NTHR;
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = T2[(long long)a * ntot + (c0 + q) * N + p];
+        if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]);
     }
 }
 
@@ -701,7 +701,7 @@ __global__ void __launch_bounds__(NTHR)
         for (int t = 0; t < PER; ++t) {
             const int e = tid + t * NTHR;
             const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
-            xold[t] = b >= 0 ? xo[b + e % N] : mk<V>((R)0, (R)0);
+            xold[t] = b >= 0 ? ld_stream(&xo[b + e % N]) : mk<V>((R)0, (R)0);
         }
     }
     __syncthreads();
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(NTHR)
         for (int t = 0; t < PER; ++t) {
             const int e = tid + t * NTHR;
             const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
-            xold[t] = b >= 0 ? xo[b + e % N] : mk<V>((R)0, (R)0);
+            xold[t] = b >= 0 ? ld_stream(&xo[b + e % N]) : mk<V>((R)0, (R)0);
         }
     }
 
@@ -799,7 +799,7 @@ __global__ void __launch_bounds__(NTHR)
             const double dr = (double)y.x - (double)x.x, di = (double)y.y - (double)x.y;
             dn = dr * dr + di * di;
         }
-        xn[idx] = y;
+        st_stream(&xn[idx], y);
         s_norm += dn;
         s_ave += (double)y.x;
         s_core += (double)dens;
