@@ -248,6 +248,47 @@ int pfdd_gl_graph_create(pfdd_gl *const *handles, int n, int iters, pfdd_gl_grap
 int pfdd_gl_graph_run(pfdd_gl_graph *g, int nsteps, double tol, pfdd_gl_reduce *out);
 int pfdd_gl_graph_destroy(pfdd_gl_graph *g);
 
+/* Conductivity sub-problem of solve_style GLCond (solve_GL+cond.cpp:70-150):
+ * a scalar complex field c (fft->condxi, host layout [i][j][k][re,im]) on
+ * the grid, advanced by one call per iteration of
+ *   forwardcond                 fft_fftw_slab.cpp:1999-2006  (FFT of c)
+ *   divcurrent                  :1704-1741   a = i c f_y/f^2, b = c f_y^2/f^2
+ *   backwardcond                :2057-2191   unnormalised inverse FFTs of a, b;
+ *                                            j = sig (El - a/eps0), rhs = (sig/eps0) b
+ *                                            + (a/eps0 - El) dsig, conductivity
+ *   update_conduct_order_param  :1794-1835   c -= rate rhs / N
+ *   prepare_next_conditr        :1866-1886   (c is read directly: no copy)
+ * It does not read the dislocation field (the reference couples the two only
+ * through the material constant rho_disloc).  The per-row layer
+ * conductivities of backwardcond are computed on the host
+ * (pfdd_host_cond_layers in include/pfdd_host.h). */
+typedef struct pfdd_gl_cond pfdd_gl_cond;
+typedef struct pfdd_gl_cond_desc {
+    int nx, ny, nz;
+    double four[9];        /* reciprocal matrix fft->four, row-major                       */
+    const double *sigma;   /* ny: condsigma of row j (backwardcond :2118-2159)             */
+    const double *dsigma;  /* ny: the condsigma_array difference of row j (:2171-2182)     */
+    double El;             /* material->El[1], applied field along y (nonzero)            */
+    double epsilonzero;    /* material->epsilonzero                                        */
+    double rate;           /* 1e-22 * app->timestep (:1821)                                */
+    int device;
+    void *stream;          /* hipStream_t; NULL = handle's own                             */
+} pfdd_gl_cond_desc;
+typedef struct pfdd_gl_cond_reduce {
+    double conductivity;        /* fft->conductivity (one rank): javerage_y.re / El      */
+    double javerage_re, javerage_im;
+    double condxinorm, condxiave;
+    double conductivity_repeat; /* what the trailing backwardcond of SolveGLCond::iterate
+                                   (solve_GL+cond.cpp:136-139) prints: backwardcond run again
+                                   on its own real-space output                          */
+} pfdd_gl_cond_reduce;
+int pfdd_gl_cond_create(const pfdd_gl_cond_desc *desc, pfdd_gl_cond **out); /* c = 0 (FFTW_Slab::allocate) */
+int pfdd_gl_cond_destroy(pfdd_gl_cond *h);
+int pfdd_gl_cond_set(pfdd_gl_cond *h, const double *c_host);
+int pfdd_gl_cond_get(pfdd_gl_cond *h, double *c_host);
+/* nsteps iterations; out receives the scalars of the last one. */
+int pfdd_gl_cond_step(pfdd_gl_cond *h, int nsteps, pfdd_gl_cond_reduce *out);
+
 #ifdef __cplusplus
 }
 #endif

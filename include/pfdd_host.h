@@ -63,8 +63,21 @@ int pfdd_host_init_random_loops(int nx, int ny, int nz, const char *app, int nlo
 
 /* The reference's hard-coded initial conditions: style "loop"
  * (initial_sxtal_loop :3146-3197), "frank_read" (:3032-3093, also fills the
- * obstacle mask xo if non-NULL), "edge" (:3201-3247), "3SBCC" (:3095-3144). */
+ * obstacle mask xo if non-NULL), "edge" (:3201-3247), "3SBCC" (:3095-3144),
+ * "StraightNonOrtho" (:2804-2855), "1LOrthoBCC" (:2600-2660, scale 1 1). */
 int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, double *xi, double *xo);
+
+/* Conduction constants of a material (material.h:44-56 defaults, then the
+ * preset's values from material.cpp; NULL or "" = defaults only) as par[11]
+ * = condsigma, mfpel, alpha_resistivity, alpha_mfpel, Rgrain, beta_disloc,
+ * rho_disloc, dlayer, Nlayers, Ngrains, pinterface. */
+int pfdd_host_cond_preset(const char *preset, double *par);
+/* Per-row conductivities of backwardcond's two-material layer stack
+ * (fft_fftw_slab.cpp:2083-2159, 2171-2182), materials A and B as par[11]
+ * above: sigma[ny] (condsigma of row j), dsigma[ny] (the bulk difference
+ * across row j).  These are pfdd_gl_cond_desc's sigma / dsigma. */
+int pfdd_host_cond_layers(const double *parA, const double *parB, int ny, double temperature, double *sigma,
+                          double *dsigma);
 
 #ifdef __cplusplus
 }

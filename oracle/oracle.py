@@ -334,3 +334,91 @@ def adjoint_rollout(xi0, BB, cddt, tau, coef, nsteps, dxi_T, amp=None, xo=None):
             gtau[a] += cddt * np.sum(mu[a].real)
         lam = lam - cddt * (g + d2 * mu.real)
     return lam, gtau, gc, states[-1]
+
+
+# ------------------------------------------------------------- conductivity
+def _c_round(x: float) -> int:
+    """C's round() for the non-negative layer counts: halves away from zero."""
+    return int(np.floor(x + 0.5)) if x >= 0 else -int(np.floor(-x + 0.5))
+
+
+def cond_layers(matA: dict, matB: dict, ny: int, temperature: float):
+    """backwardcond's per-row layer algebra (fft_fftw_slab.cpp:2083-2159) and
+    the bulk difference of its gradient term (:2171-2182): returns
+    (sigma[ny], dsigma[ny]).  Materials as dicts with the keys condsigma,
+    mfpel, alpha_resistivity, alpha_mfpel, Rgrain, beta_disloc, rho_disloc,
+    dlayer, Nlayers, Ngrains, pinterface; A is the first material command."""
+    Vf = matB["dlayer"] / (matA["dlayer"] + matB["dlayer"])
+    Nl = int(matA["Nlayers"])
+    prob = 1.0 - matA["pinterface"]
+    dT = temperature - 300
+    per = ny // Nl  # N2/(Nlayers): integer division
+    dB, dA = _c_round(per * Vf), _c_round(per * (1.0 - Vf))
+    bulk, sig = np.zeros(ny), np.zeros(ny)
+    for j in range(ny):
+        m = matB if j % (dA + dB) >= dA else matA
+        bulk[j] = m["condsigma"] / (1.0 + m["alpha_resistivity"] * dT)
+        ps = 2 * prob / m["dlayer"]
+        lam = m["mfpel"] / (1.0 + m["alpha_mfpel"] * dT)
+        ng = int(m["Ngrains"])
+        ag = (lam * ng / m["dlayer"]) * m["Rgrain"] / (1 - m["Rgrain"])
+        rhod = m["beta_disloc"] * m["rho_disloc"]
+        if ps > 0.0:
+            rhoi = (1 / bulk[j]) * ((3.0 / 16.0) * lam * ps)
+            if ng > 1:
+                rhog = (1 / bulk[j]) * (1.0 / (1.0 - 3.0 * ag / 2.0 + 3.0 * ag * ag -
+                                               3.0 * ag ** 3 * np.log(1.0 + 1.0 / ag)) - 1.0)
+                sig[j] = 1 / ((1 / bulk[j]) + (2 * rhoi + (ng - 1) * rhog) / (ng + 1) + rhod)
+            else:
+                sig[j] = 1 / ((1 / bulk[j]) + rhoi + rhod)
+        else:
+            sig[j] = bulk[j]
+    dsig = np.zeros(ny)
+    for j in range(ny):
+        if j == 0:
+            dsig[j] = bulk[j + 1] - bulk[j]
+        elif j == ny - 1:
+            dsig[j] = bulk[j] - bulk[j - 1]
+        else:
+            dsig[j] = (bulk[j + 1] - bulk[j - 1]) / 2.0
+    return sig, dsig
+
+
+def cond_step(c, four, sigma, dsigma, El, eps0, rate):
+    """One iteration of the conductivity field c (nx, ny, nz) as
+    SolveGLCond::iterate runs it: forwardcond (fft_fftw_slab.cpp:1999-2006),
+    divcurrent (:1704-1741), backwardcond (:2057-2191),
+    update_conduct_order_param (:1794-1835); then backwardcond once more on
+    its own real-space output (solve_GL+cond.cpp:136), evaluated literally by
+    a second inverse transform.  Returns (c_new, scalars)."""
+    c = np.asarray(c, dtype=np.complex128)
+    nx, ny, nz = c.shape
+    N = nx * ny * nz
+    ch = fft3d(c, -1)
+    fx, fy, fz = frec(nx, ny, nz, four)
+    f2 = fx * fx + fy * fy + fz * fz
+    pos = f2 > 0.0
+    a = np.zeros_like(ch)
+    b = np.zeros_like(ch)
+    a.real[pos] = -(ch.imag[pos] * fy[pos] / f2[pos])
+    a.imag[pos] = ch.real[pos] * fy[pos] / f2[pos]
+    b.real[pos] = ch.real[pos] * fy[pos] * fy[pos] / f2[pos]
+    b.imag[pos] = ch.imag[pos] * fy[pos] * fy[pos] / f2[pos]
+    ar, br = fft3d(a, +1), fft3d(b, +1)  # unnormalised inverse transforms
+    s = np.asarray(sigma, dtype=np.float64)[None, :, None]
+    ds = np.asarray(dsigma, dtype=np.float64)[None, :, None]
+    jr = s * (El - ar.real / eps0)
+    ji = -s * ar.imag / eps0
+    rr = (s / eps0) * br.real + (ar.real / eps0 - El) * ds
+    ri = (s / eps0) * br.imag + (ar.imag / eps0) * ds
+    cn = np.empty_like(c)
+    cn.real = c.real - rate * (rr / N)
+    cn.imag = c.imag - rate * (ri / N)
+    d = cn - c
+    a2 = fft3d(ar, +1)
+    jr2 = s * (El - a2.real / eps0)
+    out = {"javerage_re": float(jr.sum() / N), "javerage_im": float(np.broadcast_to(ji, c.shape).sum() / N),
+           "condxinorm": float(np.sqrt(np.sum(d.real ** 2 + d.imag ** 2))), "condxiave": float(cn.real.sum()),
+           "conductivity_repeat": float(np.broadcast_to(jr2, c.shape).sum() / N / El)}
+    out["conductivity"] = out["javerage_re"] / El
+    return cn, out

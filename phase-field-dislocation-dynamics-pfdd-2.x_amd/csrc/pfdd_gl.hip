@@ -1557,9 +1557,8 @@ struct Maps {
     long long n1, n2, n5;
 };
 
-Maps make_maps(const pfdd_gl *h, long long S)
+Maps make_maps_dims(long long nx, long long ny, long long nz, long long S)
 {
-    const long long nx = h->nx, ny = h->ny, nz = h->nz;
     Maps m;
     // pass 1: lines (a,i | j), FFT along k
     m.p1i = {ny * nz, 0, nz, 1, ny, 1};
@@ -1575,6 +1574,8 @@ Maps make_maps(const pfdd_gl *h, long long S)
     m.n5 = S * nx * ny;
     return m;
 }
+
+Maps make_maps(const pfdd_gl *h, long long S) { return make_maps_dims(h->nx, h->ny, h->nz, S); }
 
 // One GL iteration from xo into T2 (T2 doubles as the temporary of passes
 // 2-4), with the stress and core parameters P.
@@ -2809,6 +2810,290 @@ int pfdd_gl_graph_destroy(pfdd_gl_graph *g)
     graph_release(g);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
+    return PFDD_OK;
+}
+
+} // extern "C"
+
+// ============================================================ conductivity
+// The conductivity sub-problem of solve_style GLCond (solve_GL+cond.cpp:
+// 70-150): one scalar complex field c (fft->condxi) on the same grid, stepped
+// by forwardcond (fft_fftw_slab.cpp:1999-2006), divcurrent (:1704-1741),
+// backwardcond (:2057-2191) and update_conduct_order_param (:1794-1835).
+// The layer conductivities (backwardcond's per-row material algebra) come in
+// from the host as two arrays over the y index j: sig[j] (condsigma of row
+// j, scattering included) and dsig[j] (the difference of the bulk
+// conductivity across row j that multiplies the applied-field term).
+namespace {
+
+struct CondParams {
+    double four[9];
+    double El;       // material->El[1]
+    double eps0;     // material->epsilonzero
+    double rate;     // 1e-22 * app->timestep
+    double invN;     // 1 / (nx ny nz)
+    int nx, ny, nz;
+};
+
+// divcurrent on the transformed field in the [k][j][i] layout: a = i c f_y /
+// f^2 and b = c f_y^2 / f^2 (zero at f = 0), written as [2][k][j][i].  The
+// block also sums sig[(ny - j) mod ny] Re a over its k-points: backwardcond
+// called a second time on its own output (the trailing call of
+// SolveGLCond::iterate) transforms a back once more, and two unnormalised
+// inverse transforms give N a(-r).
+__global__ void __launch_bounds__(NTHR)
+    k_cond_div(const double2 *__restrict__ ch, double2 *__restrict__ ab, const double *__restrict__ sig, CondParams P,
+               long long N, double *__restrict__ part)
+{
+    __shared__ double red[NTHR / 64];
+    const int tid = threadIdx.x;
+    const long long idx = (long long)blockIdx.x * NTHR + tid;
+    double v[1] = {0.0};
+    if (idx < N) {
+        const int i = (int)(idx % P.nx);
+        const int j = (int)((idx / P.nx) % P.ny);
+        const int k = (int)(idx / ((long long)P.nx * P.ny));
+        const double kx = frac_freq(i, P.nx), ky = frac_freq(j, P.ny), kz = frac_freq(k, P.nz);
+        double f[3];
+        for (int c = 0; c < 3; ++c) f[c] = kx * P.four[c] + ky * P.four[3 + c] + kz * P.four[6 + c];
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double2 c = ch[idx];
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        if (f2 > 0.0) {
+            a.x = -(c.y * f[1] / f2);
+            a.y = c.x * f[1] / f2;
+            b.x = c.x * f[1] * f[1] / f2;
+            b.y = c.y * f[1] * f[1] / f2;
+        }
+        ab[idx] = a;
+        ab[N + idx] = b;
+        v[0] = sig[(P.ny - j) % P.ny] * a.x;
+    }
+    block_sum<1, NTHR>(v, red, tid);
+    if (tid == 0) part[blockIdx.x] = v[0];
+}
+
+// backwardcond's per-cell assembly on the (unnormalised) inverse transforms
+// a, b in the [2][i][j][k] layout, then update_conduct_order_param:
+//   j     = sig (El - Re a / eps0) - i sig Im a / eps0
+//   rhs   = (sig / eps0) b + (a / eps0 - El) dsig      (El on the real part)
+//   c    -= rate * rhs / N
+// Block sums: Re j, Im j, |dc|^2, Re c_new.
+__global__ void __launch_bounds__(NTHR)
+    k_cond_update(const double2 *__restrict__ ab, double2 *__restrict__ c, const double *__restrict__ sig,
+                  const double *__restrict__ dsig, CondParams P, long long N, double *__restrict__ part,
+                  long long pstride)
+{
+    __shared__ double red[4 * (NTHR / 64)];
+    const int tid = threadIdx.x;
+    const long long idx = (long long)blockIdx.x * NTHR + tid;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (idx < N) {
+        const int j = (int)((idx / P.nz) % P.ny);
+        const double2 a = ab[idx], b = ab[N + idx];
+        const double s = sig[j], ds = dsig[j];
+        const double jr = s * (P.El - a.x / P.eps0);
+        const double ji = -s * a.y / P.eps0;
+        double rr = (s / P.eps0) * b.x, ri = (s / P.eps0) * b.y;
+        rr += (a.x / P.eps0 - P.El) * ds;
+        ri += (a.y / P.eps0) * ds;
+        const double2 c0 = c[idx];
+        double2 c1;
+        c1.x = c0.x - (P.rate * (rr * P.invN));
+        c1.y = c0.y - (P.rate * (ri * P.invN));
+        c[idx] = c1;
+        const double dr = c1.x - c0.x, di = c1.y - c0.y;
+        v[0] = jr;
+        v[1] = ji;
+        v[2] = dr * dr + di * di;
+        v[3] = c1.x;
+    }
+    block_sum<4, NTHR>(v, red, tid);
+    if (tid == 0)
+        for (int q = 0; q < 4; ++q) part[q * pstride + blockIdx.x] = v[q];
+}
+
+} // namespace
+
+struct pfdd_gl_cond {
+    int device = 0, nx = 0, ny = 0, nz = 0;
+    long long N = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    CondParams P{};
+    void *tw[3] = {nullptr, nullptr, nullptr};
+    double2 *c = nullptr, *w0 = nullptr, *w1 = nullptr;
+    double *sig = nullptr, *dsig = nullptr, *part = nullptr;
+    int *done = nullptr;
+    std::vector<double> sig_h;
+    long long nb = 0;
+};
+
+namespace {
+
+void cond_release(pfdd_gl_cond *h)
+{
+    for (void *p : {(void *)h->c, (void *)h->w0, (void *)h->w1, (void *)h->sig, (void *)h->dsig, (void *)h->part,
+                    (void *)h->done, h->tw[0], h->tw[1], h->tw[2]})
+        if (p) hipFree(p);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+}
+
+// Same line maps as the GL passes, for S fields of the cond grid.
+Maps cond_maps(const pfdd_gl_cond *h, long long S) { return make_maps_dims(h->nx, h->ny, h->nz, S); }
+
+int cond_step(pfdd_gl_cond *h, pfdd_gl_cond_reduce *out)
+{
+    hipStream_t st = h->stream;
+    const Maps m1 = cond_maps(h, 1), m2 = cond_maps(h, 2);
+    const long long ncol = (long long)h->ny * h->nz;
+    const LineMap xs = {h->nx, 0, 0, 1, 1, 1};
+    // forwardcond: FFT of c (z, y, x passes) -> w0 in [k][j][i]
+    HIPCHK((launch_pass<double, false>(h->nz, h->c, h->w0, m1.p1i, m1.p1o, m1.n1, h->tw[2], h->done, st)));
+    HIPCHK((launch_pass<double, false>(h->ny, h->w0, h->w1, m1.p2i, m1.p2o, m1.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<double, false>(h->nx, h->w1, h->w0, xs, xs, ncol, h->tw[0], h->done, st)));
+    // divcurrent -> w1 = [a, b] in k space
+    const unsigned nb = (unsigned)h->nb;
+    hipLaunchKernelGGL(k_cond_div, dim3(nb), dim3(NTHR), 0, st, (const double2 *)h->w0, h->w1, h->sig, h->P, h->N,
+                       h->part + 4 * h->nb);
+    HIPCHK(hipGetLastError());
+    // backwardcond's two inverse transforms -> w0 = [a, b] in [2][i][j][k]
+    HIPCHK((launch_pass<double, true>(h->nx, h->w1, h->w0, xs, xs, 2 * ncol, h->tw[0], h->done, st)));
+    HIPCHK((launch_pass<double, true>(h->ny, h->w0, h->w1, m2.p2o, m2.p2i, m2.n2, h->tw[1], h->done, st)));
+    HIPCHK((launch_pass<double, true>(h->nz, h->w1, h->w0, m2.p5i, m2.p5o, m2.n5, h->tw[2], h->done, st)));
+    hipLaunchKernelGGL(k_cond_update, dim3(nb), dim3(NTHR), 0, st, (const double2 *)h->w0, h->c, h->sig, h->dsig,
+                       h->P, h->N, h->part, h->nb);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)5 * h->nb);
+    HIPCHK(hipMemcpyAsync(part.data(), h->part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // block partials summed in block order: deterministic
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < 5; ++q)
+        for (long long b = 0; b < h->nb; ++b) s[q] += part[(size_t)q * h->nb + b];
+    if (!std::isfinite(s[0]) || !std::isfinite(s[2]) || !std::isfinite(s[3]))
+        return fail(PFDD_ERR_NONFINITE, "non-finite conductivity reduction");
+    if (out) {
+        double mean_sig = 0.0;
+        for (double v : h->sig_h) mean_sig += v;
+        mean_sig /= (double)h->ny;
+        out->javerage_re = s[0] * h->P.invN;
+        out->javerage_im = s[1] * h->P.invN;
+        out->conductivity = out->javerage_re / h->P.El;
+        out->condxinorm = std::sqrt(s[2]);
+        out->condxiave = s[3];
+        out->conductivity_repeat = (h->P.El * mean_sig - s[4] / h->P.eps0) / h->P.El;
+    }
+    return PFDD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pfdd_gl_cond_create(const pfdd_gl_cond_desc *d, pfdd_gl_cond **out)
+{
+    if (!d || !out) return fail(PFDD_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!size_supported(d->nx) || !size_supported(d->ny) || !size_supported(d->nz))
+        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each size must be one of 8 16 32 48 64 96 128 256 512",
+                    d->nx, d->ny, d->nz);
+    if (!d->sigma || !d->dsigma) return fail(PFDD_ERR_ARG, "null layer conductivity arrays");
+    if (!(d->El != 0.0) || !(d->epsilonzero > 0.0))
+        return fail(PFDD_ERR_ARG, "need a nonzero applied field El and epsilonzero > 0");
+    pfdd_gl_cond *h = new pfdd_gl_cond;
+    auto bail = [&](int rc) {
+        cond_release(h);
+        delete h;
+        return rc;
+    };
+    h->device = d->device;
+    h->nx = d->nx;
+    h->ny = d->ny;
+    h->nz = d->nz;
+    h->N = (long long)d->nx * d->ny * d->nz;
+    h->nb = (h->N + NTHR - 1) / NTHR;
+    for (int q = 0; q < 9; ++q) h->P.four[q] = d->four[q];
+    h->P.El = d->El;
+    h->P.eps0 = d->epsilonzero;
+    h->P.rate = d->rate;
+    h->P.invN = 1.0 / (double)h->N;
+    h->P.nx = d->nx;
+    h->P.ny = d->ny;
+    h->P.nz = d->nz;
+    h->sig_h.assign(d->sigma, d->sigma + d->ny);
+    hipError_t e = hipSetDevice(d->device);
+    if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e)));
+    if (d->stream) {
+        h->stream = (hipStream_t)d->stream;
+    } else {
+        e = hipStreamCreate(&h->stream);
+        if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+        h->own_stream = true;
+    }
+    const size_t fb = sizeof(double2) * (size_t)h->N;
+    if ((e = hipMalloc((void **)&h->c, fb)) != hipSuccess || (e = hipMalloc((void **)&h->w0, 2 * fb)) != hipSuccess ||
+        (e = hipMalloc((void **)&h->w1, 2 * fb)) != hipSuccess ||
+        (e = hipMalloc((void **)&h->sig, sizeof(double) * d->ny)) != hipSuccess ||
+        (e = hipMalloc((void **)&h->dsig, sizeof(double) * d->ny)) != hipSuccess ||
+        (e = hipMalloc((void **)&h->part, sizeof(double) * 5 * h->nb)) != hipSuccess ||
+        (e = hipMalloc((void **)&h->done, sizeof(int))) != hipSuccess)
+        return bail(fail(PFDD_ERR_HIP, "hipMalloc: %s", hipGetErrorString(e)));
+    const int dims[3] = {h->nx, h->ny, h->nz};
+    for (int q = 0; q < 3; ++q) {
+        std::vector<double> tw;
+        make_twiddles(dims[q], tw);
+        if ((e = hipMalloc(&h->tw[q], sizeof(double2) * dims[q])) != hipSuccess ||
+            (e = hipMemcpy(h->tw[q], tw.data(), sizeof(double2) * dims[q], hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(fail(PFDD_ERR_HIP, "twiddles: %s", hipGetErrorString(e)));
+    }
+    const int zero = 0;
+    if ((e = hipMemcpy(h->sig, d->sigma, sizeof(double) * d->ny, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->dsig, d->dsigma, sizeof(double) * d->ny, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->done, &zero, sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(h->c, 0, fb)) != hipSuccess)
+        return bail(fail(PFDD_ERR_HIP, "upload: %s", hipGetErrorString(e)));
+    *out = h;
+    return PFDD_OK;
+}
+
+int pfdd_gl_cond_destroy(pfdd_gl_cond *h)
+{
+    if (!h) return PFDD_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    cond_release(h);
+    delete h;
+    return PFDD_OK;
+}
+
+int pfdd_gl_cond_set(pfdd_gl_cond *h, const double *c_host)
+{
+    if (!h || !c_host) return fail(PFDD_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(h->c, c_host, sizeof(double2) * h->N, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PFDD_OK;
+}
+
+int pfdd_gl_cond_get(pfdd_gl_cond *h, double *c_host)
+{
+    if (!h || !c_host) return fail(PFDD_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(c_host, h->c, sizeof(double2) * h->N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PFDD_OK;
+}
+
+int pfdd_gl_cond_step(pfdd_gl_cond *h, int nsteps, pfdd_gl_cond_reduce *out)
+{
+    if (!h) return fail(PFDD_ERR_ARG, "null handle");
+    if (nsteps < 1) return fail(PFDD_ERR_ARG, "nsteps must be >= 1");
+    HIPCHK(hipSetDevice(h->device));
+    for (int s = 0; s < nsteps; ++s) {
+        const int rc = cond_step(h, out);
+        if (rc) return rc;
+    }
     return PFDD_OK;
 }
 

@@ -39,7 +39,49 @@ void check(int rc)
 // ===================================================================== FFT_HIP
 FFT_HIP::~FFT_HIP()
 {
+    if (c_) pfdd_gl_cond_destroy(c_);
     if (h_) pfdd_gl_destroy(h_);
+}
+
+void FFT_HIP::cond_init()
+{
+    // FFTW_Slab::allocate's condxi (zero) and backwardcond's layer algebra,
+    // which needs two materials (fft_fftw_slab.cpp:2083)
+    if (!have_matB) deck_error("Need two materials for this app!");
+    std::string msg;
+    if (!cond_layers(mat, matB, ny, temperature, cond_sigma, cond_dsigma, &msg)) deck_error(msg);
+    pfdd_gl_cond_desc d{};
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) d.four[3 * i + j] = geo.four[i][j];
+    d.sigma = cond_sigma.data();
+    d.dsigma = cond_dsigma.data();
+    d.El = mat.El[1];
+    d.epsilonzero = mat.epsilonzero;
+    d.rate = 1e-22 * timestep;
+    d.device = 0;
+    check(pfdd_gl_cond_create(&d, &c_));
+    const double Vf = matB.dlayer / (mat.dlayer + matB.dlayer);
+    printf("temperature: %f \n", temperature);
+    printf("size in y: %i Nmaterials: %i \n", ny, 2);
+    printf("materialA: conductivity = %f MS/m, mean-free-path = %f nm, dlayer = %f nm\n", mat.condsigma / 1e6,
+           1.e9 * mat.mfpel, 1.e9 * mat.dlayer);
+    printf("materialB: conductivity = %f MS/m, mean-free-path = %f nm, dlayer = %f nm\n", matB.condsigma / 1e6,
+           1.e9 * matB.mfpel, 1.e9 * matB.dlayer);
+    printf("Nlayers: %i, Vf = %f \n", mat.Nlayers, Vf);
+}
+
+void FFT_HIP::cond_iterate()
+{
+    check(pfdd_gl_cond_step(c_, 1, &cond_last));
+}
+
+void FFT_HIP::cond_get(std::vector<double> &c)
+{
+    c.assign((size_t)2 * nx * ny * nz, 0.0);
+    check(pfdd_gl_cond_get(c_, c.data()));
 }
 
 void FFT_HIP::init()
@@ -381,7 +423,13 @@ void App::init_app(FFT_HIP &fft)
     const long long n = (long long)g.S * g.N();
     fft.xi.assign(2 * n, 0.0);
     fft.xo.assign(n, 0.0);
-    if (init_style == "loops") {
+    // the app's default setup_flag when the deck gives no initial command
+    if (!init_set && !fft.slip.default_init.empty()) init_style = fft.slip.default_init;
+    if (init_style == "StraightNonOrtho") {
+        init_sxtal_straight_nonortho(g, fft.xi.data());
+    } else if (init_style == "1LOrthoBCC") {
+        init_sxtal_1lorthobcc(g, scale[0], scale[1], fft.xi.data());
+    } else if (init_style == "loops") {
         init_random_loops(g, fft.slip, init_nloops, init_rmin, init_rmax, init_noise, seed, fft.xi.data());
     } else if (init_style == "loop") {
         init_sxtal_loop(g, fft.xi.data());
@@ -445,6 +493,39 @@ void SolveGL::iterate(App &app, FFT_HIP &fft, Output &out)
                 printf("Loading step %d converged after %d iterations\n", sstate, it_last);
                 break;
             }
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) fft.sigma[i][j] = fft.sigma[i][j] + fft.deltasig[i][j];
+    }
+    out.loop_time = wall() - t0;
+}
+
+void SolveGL::iterate_cond(App &app, FFT_HIP &fft, Output &out)
+{
+    // SolveGLCond::iterate (solve_GL+cond.cpp:70-150) as it executes: the
+    // body of the iteration loop ends in an unconditional break (the if of
+    // :134 guards only prep_backward), so every stress state runs one
+    // iteration -- the dislocation step and the conductivity step -- and
+    // then prints backwardcond's conductivity once more, computed by
+    // backwardcond on its own output.
+    const double t0 = wall();
+    printf("Beginning Time Evolution\n");
+    for (int sstate = 0; sstate < app.stopsteps; ++sstate) {
+        fft.rotate_stress();
+        fft.resolSS();
+        for (int it = 0; it < max_iter; ++it) {
+            fft.cond_iterate();
+            printf("\nconductivity_local: %f \n", fft.cond_last.conductivity);
+            printf("\nconductivity: %f \n", fft.cond_last.conductivity);
+            fft.iterate_block(1, 0.0);
+            ++gpu_calls;
+            out.loop_time = wall() - t0;
+            if (app.time >= app.nextoutput) app.nextoutput = out.compute(app, fft, app.time, 0);
+            app.time += 1;
+            app.steps += 1;
+            printf("\nconductivity_local: %f \n", fft.cond_last.conductivity_repeat);
+            printf("\nconductivity: %f \n", fft.cond_last.conductivity_repeat);
+            break;
         }
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) fft.sigma[i][j] = fft.sigma[i][j] + fft.deltasig[i][j];
@@ -546,8 +627,13 @@ void Input::command(const std::vector<std::string> &w)
         if (!box) deck_error("Cannot create sites before simulation box is defined");
         sites = true;
     } else if (c == "material") {
-        // Material::Material (material.cpp:32-167)
-        Material &m = fft.mat;
+        // Material::Material (material.cpp:32-167).  FFT::set_material
+        // (fft.cpp:221-246): the first material command is fft->material
+        // (materialA), the second materialB (the conductivity layers); later
+        // ones are parsed and kept by nobody on this path
+        Material spare;
+        Material &m = !have_material ? fft.mat : !fft.have_matB ? fft.matB : spare;
+        const bool second = have_material && !fft.have_matB;
         for (size_t i = 1; i < w.size(); ++i) {
             const std::string &k = w[i];
             if (k == "preset") {
@@ -564,9 +650,19 @@ void Input::command(const std::vector<std::string> &w)
                 while (fscanf(f, "%d %d %d %d %lf", &a, &b, &cc, &s, &v) == 5)
                     m.A_mpea[(size_t)a * n2 * n3 + (size_t)b * n3 + cc + (size_t)s * n1 * n2 * n3] = v * 1e-3;
                 fclose(f);
+            } else if (k == "El") {
+                for (int q = 0; q < 3; ++q) m.El[q] = num(w, ++i, "material");
             } else {
                 const double v = num(w, ++i, "material");
                 if (k == "a") m.a = v;
+                else if (k == "dlayer") m.dlayer = v;
+                else if (k == "Nlayers") m.Nlayers = (int)v;
+                else if (k == "Ngrains") m.Ngrains = (int)v;
+                else if (k == "Rgrain") m.Rgrain = v;
+                else if (k == "alpha_rho") m.alpha_resistivity = v, m.alpha_mfpel = v;
+                else if (k == "pinterface") m.pinterface = v;
+                else if (k == "rho_disloc") m.rho_disloc = v;
+                else if (k == "beta_disloc") m.beta_disloc = v;
                 else if (k == "b") m.b = v;
                 else if (k == "mu") m.mu = v;
                 else if (k == "young") m.young = v;
@@ -590,9 +686,11 @@ void Input::command(const std::vector<std::string> &w)
         if (m.b == 0.0 && m.a > 0.0) m.b = m.a / std::sqrt(2.0);
         if (!(m.mu > 0) || !(m.young > 0)) deck_error("material needs mu and young (or a preset)");
         m.derive();
+        if (second) fft.have_matB = true;
         have_material = true;
     } else if (c == "solve_style") {
-        if (w.size() < 2 || w[1] != "GL") deck_error("Invalid solve style");
+        if (w.size() < 2 || (w[1] != "GL" && w[1] != "GLCond")) deck_error("Invalid solve style");
+        solve.cond = fft.cond = w[1] == "GLCond";
         for (size_t i = 2; i < w.size(); ++i) {
             if (w[i] == "max_iter") solve.max_iter = (int)num(w, ++i, "solve");
             else if (w[i] == "tol") solve.tol = num(w, ++i, "solve");
@@ -604,7 +702,7 @@ void Input::command(const std::vector<std::string> &w)
         else if (w[1] == "stress") output.diag_stress = true;
         else deck_error("Invalid diag_style " + w[1]);
     } else if (c == "temperature") {
-        num(w, 1, "temperature");
+        fft.temperature = num(w, 1, "temperature"); // app->temperature
     } else if (c == "non_Schmid") {
         // App::set_nonSchmid (app.cpp:446-460): angle_to_110 w1 w2 w3
         if (w.size() != 5) deck_error("Illegal non_Schmid command");
@@ -659,6 +757,7 @@ void Input::command(const std::vector<std::string> &w)
     } else if (c == "initial") {
         // extension: seeded initial fields
         if (w.size() < 2) deck_error("Illegal initial command");
+        app.init_set = true;
         if (w[1] == "loops") {
             app.init_style = "loops";
             app.init_nloops = (int)num(w, 2, "initial");
@@ -746,6 +845,7 @@ void Input::run(const std::vector<std::string> &w)
             return;
         }
         fft.init();
+        if (fft.cond) fft.cond_init();
     } else {
         app.stopsteps = (int)std::floor(app.stoptime / fft.timestep);
         if (check_only) return;
@@ -753,7 +853,8 @@ void Input::run(const std::vector<std::string> &w)
     printf("Setting up run ...\n");
     fft.rotate_stress(); // FFTW_Slab::setup (fft_fftw_slab.cpp:111-119)
     app.nextoutput = output.setup(app, fft, app.time);
-    solve.iterate(app, fft, output);
+    if (solve.cond) solve.iterate_cond(app, fft, output);
+    else solve.iterate(app, fft, output);
     printf("Loop time of %g on %d procs\n", output.loop_time, 1);
     if (!write_xi_file.empty()) {
         fft.sync_host();

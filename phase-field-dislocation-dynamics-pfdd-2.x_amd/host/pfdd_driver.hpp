@@ -78,8 +78,23 @@ class FFT_HIP {
     void upload_xi();     // host mirror -> device
     pfdd_gl *handle() const { return h_; }
 
+    // conductivity sub-problem of solve_style GLCond: the second material
+    // command (fft->materialB), the temperature command, the per-row layer
+    // conductivities, and the GPU field c (fft->condxi)
+    Material matB;
+    bool have_matB = false, cond = false;
+    double temperature = 300.0;
+    std::vector<double> cond_sigma, cond_dsigma;
+    pfdd_gl_cond_reduce cond_last{};
+    void cond_init();
+    // forwardcond + divcurrent + backwardcond + update_conduct_order_param
+    // (+ prepare_next_conditr) of one iteration
+    void cond_iterate();
+    void cond_get(std::vector<double> &c);
+
   private:
     pfdd_gl *h_ = nullptr;
+    pfdd_gl_cond *c_ = nullptr;
 };
 
 struct DumpSpec {
@@ -118,6 +133,7 @@ class App {
     double sigma0[3][3] = {}, dsigma[3][3] = {};
     // initial field
     std::string init_style = "loops";
+    bool init_set = false; // an initial command was given (else the app's default setup)
     int init_nloops = 0, seed = 576;
     double init_rmin = 8, init_rmax = 20, init_noise = 0.0;
     struct Disc {
@@ -133,7 +149,10 @@ class SolveGL {
     int max_iter = 10;
     double tol = 0.0;
     long long gpu_calls = 0;
+    bool cond = false; // solve_style GLCond
     void iterate(App &app, FFT_HIP &fft, Output &out);
+    // SolveGLCond::iterate (solve_GL+cond.cpp:70-150) as executed
+    void iterate_cond(App &app, FFT_HIP &fft, Output &out);
 };
 
 // Parses and runs an in.* deck.  Throws DeckError on a malformed deck (the

@@ -14,7 +14,7 @@ bool Material::preset(const std::string &name)
     // set_constants, material.cpp:176-835: elastic constants, Burgers vector
     // rule, sine-core (isf, usf, An), extended gamma-surface (c0..c4, a1, a3)
     // and pyramidal-II Fourier (aa0..aa4, bb1..bb4) constants of each named
-    // preset.  Conductivity constants are not part of the GL path.
+    // preset, and the conduction constants of those that have them.
     // b: 0 = a/sqrt2 (fcc), 1 = sqrt3 a/2 (bcc), 2 = given, 3 = not set
     struct Row {
         const char *name;
@@ -115,6 +115,27 @@ bool Material::preset(const std::string &name)
         // {1} marks the Mg pyramidal-II constants the Mg alloys reuse
         const double *py = (r.pyr[0] == 1 && r.pyr[1] == 0) ? MG_PYR : r.pyr;
         for (int q = 0; q < 9; ++q) pyr[q] = py[q];
+        // conduction constants of the presets that have them (condsigma,
+        // mfpel, alpha_resistivity, alpha_mfpel, Rgrain, beta_disloc)
+        static const struct {
+            const char *name;
+            double v[6];
+        } cond[] = {{"Ni", {1.43e7, 5.87e-9, 0.006, 0.006, 0.19, 10.e-25}},
+                    {"Al", {3.77e7, 18.9e-9, 0.00429, 0.00429, 0.17, 1.5e-25}},
+                    {"Au", {4.11e7, 37.7e-9, 0.0034, 0.0034, 0.17, 2.6e-25}},
+                    {"Cu", {5.95e7, 39.9e-9, 0.00385, 0.00385, 0.24, 1.3e-25}},
+                    {"Ag", {6.3e7, 53.3e-9, 0.0038, 0.0038, 0.24, 1.9e-25}},
+                    {"Ta", {7.634e6, 3.21e-9, 0.0035, 0.0035, 0.1, 0}},
+                    {"W", {1.855e7, 15.5e-9, 0.0042, 0.0042, 0.42, 7.5e-25}},
+                    {"Cr", {7.937e6, 2.9e-9, 0.003, 0.003, 0.1, 0}},
+                    {"Fe", {1.041e6, 40.0e-9, 0.005, 0.005, 0.1, 10.e-25}},
+                    {"Nb", {6.334e6, 2.0e-9, 0.00318, 0.00318, 0.45, 0}}};
+        for (const auto &cr : cond)
+            if (name == cr.name) {
+                condsigma = cr.v[0], mfpel = cr.v[1], alpha_resistivity = cr.v[2], alpha_mfpel = cr.v[3];
+                Rgrain = cr.v[4];
+                if (cr.v[5] != 0) beta_disloc = cr.v[5];
+            }
         // USFE(angle, tau) fits of core flag 6 (material.cpp:713-717, 766-770)
         if (name == "Ta") a_slope = 51.59816407, a_b = -2.27173784, b_b = 0.9979452, c_slope = -4.16536, c_b = 1133.961364;
         if (name == "Fe")
@@ -245,6 +266,33 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         }
         return true;
     }
+    if (app == "1L1S100B" || app == "cond100B") {
+        // App1L1S100B (app_1L1S_100B.cpp:29-49, 215-229) and Appcond100B
+        // (app_cond_100B.cpp:29-49, 90-104): one screw system, b = x on the
+        // plane n = z, no rotation block; default setup StraightNonOrtho
+        t.slip_systems = 1;
+        t.num_planes = 1;
+        t.CD = 0.5;
+        t.default_core = 1;
+        t.default_init = "StraightNonOrtho";
+        t.rotate = false;
+        t.xn = {0.0, 0.0, 1.0};
+        t.xb = {1.0, 0.0, 0.0};
+        return true;
+    }
+    if (app == "2d-bcc-1lOrtho") {
+        // App2Dbcc1LORTHO (app_2d_bcc_ortho_1loop.cpp:29-49, 210-256):
+        // n = (1, -1, 0)/sqrt2, b = (1, 1, 1)/2, rotated by four/normfour;
+        // default setup 1LOrthoBCC
+        t.slip_systems = 1;
+        t.num_planes = 1;
+        t.CD = 0.5;
+        t.default_core = 1;
+        t.default_init = "1LOrthoBCC";
+        t.xn = {1.0 / std::sqrt(2), -1.0 / std::sqrt(2), 0.0};
+        t.xb = {0.5, 0.5, 0.5};
+        return true;
+    }
     if (app == "frank_read") {
         // AppFrankRead (app_frank_read.cpp:29-50, 96-120)
         t.slip_systems = 2;
@@ -359,6 +407,59 @@ std::vector<double> core_coefficients(const Material &m, int flag)
     }
     return {};
 }
+
+bool cond_layers(const Material &A, const Material &B, int ny, double temperature, std::vector<double> &sigma,
+                 std::vector<double> &dsigma, std::string *msg)
+{
+    // backwardcond, fft_fftw_slab.cpp:2083-2159: alternating layers A | B
+    // along y, starting with A; integer division N2/Nlayers as written
+    const double Vf = B.dlayer / (A.dlayer + B.dlayer);
+    const int Nl = A.Nlayers;
+    const double prob = 1.0 - A.pinterface;
+    const double dT = temperature - 300;
+    if (Nl < 1 || ny < 2) {
+        if (msg) *msg = "conductivity needs Nlayers >= 1 and ny >= 2";
+        return false;
+    }
+    const int dB = (int)std::round((double)(ny / Nl) * Vf);
+    const int dA = (int)std::round((double)(ny / Nl) * (1. - Vf));
+    if (dA + dB < 1) {
+        if (msg) *msg = "layer thicknesses round to zero rows";
+        return false;
+    }
+    std::vector<double> bulk(ny);
+    sigma.assign(ny, 0.0);
+    for (int j = 0; j < ny; ++j) {
+        const Material &m = (j % (dA + dB) >= dA) ? B : A;
+        bulk[j] = m.condsigma / (1. + m.alpha_resistivity * dT);
+        const double PSprob = 2 * prob / m.dlayer;
+        const double lam = m.mfpel / (1. + m.alpha_mfpel * dT);
+        const double ag = (lam * m.Ngrains / m.dlayer) * m.Rgrain / (1 - m.Rgrain);
+        const double rhod = m.beta_disloc * m.rho_disloc;
+        double cs;
+        if (PSprob > 0.) {
+            const double rhoi = (1 / bulk[j]) * ((3. / 16.) * lam * PSprob);
+            if (m.Ngrains > 1) {
+                const double rhog =
+                    (1 / bulk[j]) * (1. / (1. - 3. * ag / 2. + 3. * ag * ag - 3. * std::pow(ag, 3.) * std::log(1. + 1. / ag)) - 1.);
+                cs = 1 / ((1 / bulk[j]) + (2 * rhoi + (m.Ngrains - 1) * rhog) / (m.Ngrains + 1) + rhod);
+            } else {
+                cs = 1 / ((1 / bulk[j]) + rhoi + rhod);
+            }
+        } else {
+            cs = bulk[j];
+        }
+        sigma[j] = cs;
+    }
+    dsigma.assign(ny, 0.0);
+    for (int j = 0; j < ny; ++j) {
+        if (j == 0) dsigma[j] = bulk[j + 1] - bulk[j];
+        else if (j == ny - 1) dsigma[j] = bulk[j] - bulk[j - 1];
+        else dsigma[j] = (bulk[j + 1] - bulk[j - 1]) / 2.;
+    }
+    return true;
+}
+
 
 // -------------------------------------------------------------------- RANMAR
 RanMars::RanMars(int seed)
@@ -506,6 +607,50 @@ void init_sxtal_edge(const Grid &g, double *xi)
                     const long long idx = k + (long long)j * g.nz + (long long)i * g.ny * g.nz + (long long)is * N;
                     const bool on = is == 0 && i == 0 &&
                                     ((k <= 3 * j && (k + g.nz / 2 > 3 * j)) || (k - g.nz / 2) > 3 * j);
+                    xi[2 * idx] = on ? 1.0 : 0.0;
+                    xi[2 * idx + 1] = 0.0;
+                }
+}
+
+void init_sxtal_straight_nonortho(const Grid &g, double *xi)
+{
+    // initial_sxtal_StraightNonOrtho (fft_fftw_slab.cpp:2804-2855): system 0,
+    // the plane k = nz/2 (|k - nz/2| < 0.1), cells with 2i - j <= -nx or
+    // 0 <= 2i - j <= nx
+    const long long N = g.N();
+    const double eps = 1e-1;
+    for (int is = 0; is < g.S; ++is)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    const long long idx = k + (long long)j * g.nz + (long long)i * g.ny * g.nz + (long long)is * N;
+                    const double dk = k - g.nz / 2.0;
+                    const int d = 2 * i - j;
+                    const bool on = dk < eps && dk > -eps && (d <= -g.nx || (d >= 0 && d <= g.nx)) && is == 0;
+                    xi[2 * idx] = on ? 1.0 : 0.0;
+                    xi[2 * idx + 1] = 0.0;
+                }
+}
+
+void init_sxtal_1lorthobcc(const Grid &g, double sa, double sb, double *xi)
+{
+    // initial_sxtal_1LOrthoBCC (fft_fftw_slab.cpp:2600-2660): system 0, the
+    // plane k = nz/2, a disc of radius 10 sa about (nx/2, ny/2) in the
+    // scaled coordinates (sa (i - nx/2), sb (j - ny/2))
+    const long long N = g.N();
+    const double eps = 1e-1;
+    for (int is = 0; is < g.S; ++is)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    const long long idx = k + (long long)j * g.nz + (long long)i * g.ny * g.nz + (long long)is * N;
+                    const double dk = k - (g.nz / 2.0);
+                    bool on = false;
+                    if (dk < eps && dk > -eps && is == 0) {
+                        const double ir = std::sqrt(sa * (i - g.nx / 2.0) * sa * (i - g.nx / 2.0) +
+                                                    sb * (j - g.ny / 2.0) * sb * (j - g.ny / 2.0));
+                        on = ir <= 10 * sa;
+                    }
                     xi[2 * idx] = on ? 1.0 : 0.0;
                     xi[2 * idx + 1] = 0.0;
                 }

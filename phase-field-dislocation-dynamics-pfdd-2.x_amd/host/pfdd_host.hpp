@@ -18,6 +18,13 @@ struct Material {
     double pyr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // aa0..aa4, bb1..bb4 (pyramidal II)
     double a_slope = 0, a_b = 0, b_b = 0, c_slope = 0, c_b = 0; // USFE(angle, tau) fit (flag 6)
     double dslip = 0.8165;
+    // electrical conduction (material.h:44-56, presets of material.cpp), read by
+    // the conductivity sub-problem of solve_style GLCond
+    double condsigma = 0, alpha_resistivity = 0, Rgrain = 0, mfpel = 1e-9, alpha_mfpel = 0;
+    double dlayer = 100e-9, pinterface = 0.5, epsilonzero = 8.8541878128e-12;
+    double beta_disloc = 0, rho_disloc = 0;
+    int Nlayers = 1, Ngrains = 1;
+    double El[3] = {0, 0, 0};
     // derived (material.cpp:156-167)
     double C44 = 0, C12 = 0, C11 = 0, nu = 0, ll = 0, S11 = 0, S12 = 0, S44 = 0;
     // Voigt re-derivation stored by Bmatrix (fft_fftw_slab.cpp:487-496)
@@ -50,12 +57,14 @@ struct SlipTable {
     double CD = 0.5;       // dislocation mobility (app ctor)
     double timestep = 1.0; // app->timestep
     int default_core = 1;
+    std::string default_init; // the app's default setup_flag as an initial style ("" = none built)
     bool rotate = true;         // set_slip applies the four/normfour rotation
     std::vector<double> xn, xb; // S*3 each
 };
 
-// Known app styles: fcc, 3d-hcp-basal, frank_read, 3SBCC-like "fcc111-3"
-// (three <110> on one {111} plane).  Returns false for unknown names.
+// Known app styles: fcc, 3d-hcp-basal, frank_read, 1L1S100B, cond100B,
+// 2d-bcc-1lOrtho, and basal3 (three unit b at 0/120/240 deg on the plane
+// n = z).  Returns false for unknown names.
 bool slip_table(const std::string &app, SlipTable &t, double oflag = 0.0);
 
 // AppFCC::set_slip rotation block (app_fcc.cpp:288-313).
@@ -75,6 +84,15 @@ double usfe_angle_tau(const Material &m, const double sigma_rot[3][3], double an
 // 1165-1178, 1396-1406): flag 1 -> {An'}, flag 3 -> {isf', Cn'} with Cn' =
 // (usf - isf'/2)/(mu dslip b), the reference's own expression.
 std::vector<double> core_coefficients(const Material &m, int flag);
+
+// Per-row conductivity of the two-material layer stack of backwardcond
+// (fft_fftw_slab.cpp:2083-2159, rows j along y): A = the first material
+// command (fft->material), B = the second; sigma[j] = condsigma of row j
+// (bulk / (1 + alpha dT) with interface, grain-boundary and dislocation
+// resistivities added), dsigma[j] = the difference of the bulk values across
+// row j (:2171-2182).  Returns false (and leaves msg) on a stack with no rows.
+bool cond_layers(const Material &A, const Material &B, int ny, double temperature, std::vector<double> &sigma,
+                 std::vector<double> &dsigma, std::string *msg = nullptr);
 
 // RanMars (random_mars.cpp:41-101).
 class RanMars {
@@ -104,6 +122,10 @@ void init_sxtal_loop(const Grid &g, double *xi);
 void init_sxtal_frank_read(const Grid &g, double *xi, double *xo);
 // initial_sxtal_edge (fft_fftw_slab.cpp:3201-3247).
 void init_sxtal_edge(const Grid &g, double *xi);
+// initial_sxtal_StraightNonOrtho (fft_fftw_slab.cpp:2804-2855).
+void init_sxtal_straight_nonortho(const Grid &g, double *xi);
+// initial_sxtal_1LOrthoBCC (fft_fftw_slab.cpp:2600-2660); sa, sb = scale command.
+void init_sxtal_1lorthobcc(const Grid &g, double sa, double sb, double *xi);
 // initial_sxtal_3SBCC (fft_fftw_slab.cpp:3095-3144).
 void init_sxtal_3sbcc(const Grid &g, double *xi);
 

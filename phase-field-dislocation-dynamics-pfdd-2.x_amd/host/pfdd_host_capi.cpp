@@ -165,6 +165,8 @@ int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, doub
     else if (s == "frank_read") init_sxtal_frank_read(g, xi, xo);
     else if (s == "edge") init_sxtal_edge(g, xi);
     else if (s == "3SBCC") init_sxtal_3sbcc(g, xi);
+    else if (s == "StraightNonOrtho") init_sxtal_straight_nonortho(g, xi);
+    else if (s == "1LOrthoBCC") init_sxtal_1lorthobcc(g, 1.0, 1.0, xi);
     else return 1;
     if (xo && s != "frank_read")
         for (long long q = 0; q < (long long)ns * g.N(); ++q) xo[q] = 0.0;
@@ -172,3 +174,34 @@ int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, doub
 }
 
 } // extern "C"
+
+static void cond_params(const double *p, Material &m)
+{
+    m.condsigma = p[0], m.mfpel = p[1], m.alpha_resistivity = p[2], m.alpha_mfpel = p[3], m.Rgrain = p[4];
+    m.beta_disloc = p[5], m.rho_disloc = p[6], m.dlayer = p[7], m.Nlayers = (int)p[8], m.Ngrains = (int)p[9];
+    m.pinterface = p[10];
+}
+
+extern "C" int pfdd_host_cond_preset(const char *preset, double *par)
+{
+    if (!par) return 1;
+    Material m;
+    if (preset && *preset && !m.preset(preset)) return 1;
+    const double v[11] = {m.condsigma, m.mfpel, m.alpha_resistivity, m.alpha_mfpel, m.Rgrain, m.beta_disloc,
+                          m.rho_disloc, m.dlayer, (double)m.Nlayers, (double)m.Ngrains, m.pinterface};
+    for (int q = 0; q < 11; ++q) par[q] = v[q];
+    return 0;
+}
+
+extern "C" int pfdd_host_cond_layers(const double *parA, const double *parB, int ny, double temperature,
+                                     double *sigma, double *dsigma)
+{
+    if (!parA || !parB || !sigma || !dsigma) return 1;
+    Material A, B;
+    cond_params(parA, A);
+    cond_params(parB, B);
+    std::vector<double> s, d;
+    if (!cond_layers(A, B, ny, temperature, s, d)) return 1;
+    for (int j = 0; j < ny; ++j) sigma[j] = s[j], dsigma[j] = d[j];
+    return 0;
+}

@@ -66,6 +66,27 @@ class GLReduce(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CondDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("four", ctypes.c_double * 9),
+        ("sigma", _dp), ("dsigma", _dp),
+        ("El", ctypes.c_double), ("epsilonzero", ctypes.c_double), ("rate", ctypes.c_double),
+        ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
+    ]
+
+
+class CondReduce(ctypes.Structure):
+    _fields_ = [
+        ("conductivity", ctypes.c_double), ("javerage_re", ctypes.c_double), ("javerage_im", ctypes.c_double),
+        ("condxinorm", ctypes.c_double), ("condxiave", ctypes.c_double),
+        ("conductivity_repeat", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = [
     "pfdd_gl_version", "pfdd_gl_last_error", "pfdd_gl_create", "pfdd_gl_destroy",
     "pfdd_gl_set_xi", "pfdd_gl_get_xi", "pfdd_gl_xi_device", "pfdd_gl_set_xi_device",
@@ -75,6 +96,7 @@ EXPORTS = [
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
     "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
     "pfdd_gl_autotune", "pfdd_gl_set_tuning",
+    "pfdd_gl_cond_create", "pfdd_gl_cond_destroy", "pfdd_gl_cond_set", "pfdd_gl_cond_get", "pfdd_gl_cond_step",
 ]
 
 _lib = None
@@ -118,6 +140,11 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_graph_destroy": (c_int, [vp]),
         "pfdd_gl_autotune": (c_int, [vp, c_int, ip]),
         "pfdd_gl_set_tuning": (c_int, [vp, ip]),
+        "pfdd_gl_cond_create": (c_int, [ctypes.POINTER(CondDesc), ctypes.POINTER(vp)]),
+        "pfdd_gl_cond_destroy": (c_int, [vp]),
+        "pfdd_gl_cond_set": (c_int, [vp, _dp]),
+        "pfdd_gl_cond_get": (c_int, [vp, _dp]),
+        "pfdd_gl_cond_step": (c_int, [vp, c_int, ctypes.POINTER(CondReduce)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -327,6 +354,51 @@ class GLGraph:
         if getattr(self, "_g", None):
             self._lib.pfdd_gl_graph_destroy(self._g)
             self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CondSolver:
+    """Conductivity field of solve_style GLCond on one GPU (pfdd_gl_cond_*):
+    c has shape (nx, ny, nz), complex128; sigma, dsigma are the per-row layer
+    conductivities (pfdd_host_cond_layers)."""
+
+    def __init__(self, shape, four, sigma, dsigma, El, epsilonzero=8.8541878128e-12, rate=1e-22, device=0):
+        L = load_library()
+        self.shape = tuple(int(v) for v in shape)
+        self._sig, self._dsig = _f64(sigma), _f64(dsigma)
+        d = CondDesc()
+        d.nx, d.ny, d.nz = self.shape
+        d.four[:] = _f64(four).reshape(-1).tolist()
+        d.sigma, d.dsigma = _ptr(self._sig), _ptr(self._dsig)
+        d.El, d.epsilonzero, d.rate = float(El), float(epsilonzero), float(rate)
+        d.device = device
+        h = ctypes.c_void_p()
+        _check(L.pfdd_gl_cond_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h, self._lib = h, L
+
+    def set_c(self, c):
+        a = np.ascontiguousarray(np.asarray(c, dtype=np.complex128).reshape(self.shape))
+        _check(self._lib.pfdd_gl_cond_set(self._h, a.view(np.float64).ctypes.data_as(_dp)))
+
+    def get_c(self):
+        a = np.empty(self.shape, dtype=np.complex128)
+        _check(self._lib.pfdd_gl_cond_get(self._h, a.view(np.float64).ctypes.data_as(_dp)))
+        return a
+
+    def step(self, nsteps=1):
+        r = CondReduce()
+        _check(self._lib.pfdd_gl_cond_step(self._h, int(nsteps), ctypes.byref(r)))
+        return r.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pfdd_gl_cond_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

@@ -43,6 +43,8 @@ def lib():
             "pfdd_host_ranmar_fill": [ci, cll, _dp],
             "pfdd_host_init_random_loops": [ci, ci, ci, cp, ci, cd, cd, cd, ci, _dp, _dp],
             "pfdd_host_init_sxtal": [cp, ci, ci, ci, ci, _dp, _dp],
+            "pfdd_host_cond_preset": [cp, _dp],
+            "pfdd_host_cond_layers": [_dp, _dp, ci, cd, _dp, _dp],
         }.items():
             f = getattr(L, name)
             f.restype = ci
@@ -128,6 +130,28 @@ def sxtal(style, shape, S, with_mask=False):
     _ok(lib().pfdd_host_init_sxtal(style.encode(), nx, ny, nz, S, xi.ctypes.data_as(_dp), _p(xo)),
         "init " + style)
     return (xi, xo) if with_mask else xi
+
+
+COND_KEYS = ("condsigma", "mfpel", "alpha_resistivity", "alpha_mfpel", "Rgrain", "beta_disloc", "rho_disloc",
+             "dlayer", "Nlayers", "Ngrains", "pinterface")
+
+
+def cond_material(preset=None, **over):
+    """Conduction constants of a material (defaults, preset, then overrides) as a dict."""
+    par = np.zeros(11)
+    _ok(lib().pfdd_host_cond_preset((preset or "").encode(), _p(par)), f"cond preset {preset}")
+    d = dict(zip(COND_KEYS, par.tolist()))
+    d.update(over)
+    return d
+
+
+def cond_layers(matA, matB, ny, temperature=300.0):
+    """Per-row (sigma, dsigma) of the two-material layer stack (host C++)."""
+    pa = np.array([float(matA[k]) for k in COND_KEYS])
+    pb = np.array([float(matB[k]) for k in COND_KEYS])
+    sig, dsig = np.zeros(ny), np.zeros(ny)
+    _ok(lib().pfdd_host_cond_layers(_p(pa), _p(pb), int(ny), float(temperature), _p(sig), _p(dsig)), "cond layers")
+    return sig, dsig
 
 
 def fcc_loops_problem(shape, nloops=8, rmin=8, rmax=20, noise=1e-3, shear_pa=400e6, seed=576,

@@ -1,0 +1,229 @@
+"""Conductivity sub-problem of solve_style GLCond.
+
+CPU: the oracle's restatement (oracle.cond_layers, oracle.cond_step) pinned by
+closed forms -- Matthiessen sums of the layer resistivities, the zero field,
+a single plane wave through the FFT sign and frequency conventions, and the
+second inverse transform of the trailing backwardcond as an index reflection
+-- and the host C++ layer algebra against the oracle.
+GPU (pfdd_gl_cond_*, through the C ABI): fields and scalars within 1e-9 of
+the oracle over several iterations; the reference's conductivity deck.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "phase-field-dislocation-dynamics-pfdd-2.x_amd", "lib", "pfdd_hip")
+
+# the two materials of the reference's conductivity deck (in.CuNb-pfdd:11-12)
+CU = dict(condsigma=5.95e7, mfpel=39.9e-9, alpha_resistivity=0.00385, alpha_mfpel=0.00385, Rgrain=0.83,
+          beta_disloc=1.3e-25, rho_disloc=1e15, dlayer=1e-7, Nlayers=1, Ngrains=1, pinterface=0.5)
+NB = dict(condsigma=6.334e6, mfpel=2e-9, alpha_resistivity=0.00318, alpha_mfpel=0.00318, Rgrain=0.45,
+          beta_disloc=0.0, rho_disloc=1e15, dlayer=1.1304347826086957e-08, Nlayers=1, Ngrains=1, pinterface=0.5)
+
+
+def test_layer_conductivity_matthiessen_sums():
+    """sigma of a row is the bulk value with resistivities added (Matthiessen's
+    rule): pinterface = 1 adds none; the interface term is (3/16) lam PS /
+    sigma_bulk with PS = 2 (1 - p) / d; the dislocation term beta rho adds
+    linearly; a temperature shift divides the bulk value by 1 + alpha dT."""
+    A = dict(CU, pinterface=1.0, rho_disloc=0.0)
+    s, d = oracle.cond_layers(A, A, 32, 300.0)
+    assert np.allclose(s, A["condsigma"], rtol=1e-15) and np.all(d == 0)
+    s, _ = oracle.cond_layers(CU, CU, 32, 300.0)
+    lam, ps = CU["mfpel"], 2 * (1 - CU["pinterface"]) / CU["dlayer"]
+    rho = 1 / CU["condsigma"] + (3.0 / 16.0) * lam * ps / CU["condsigma"] + CU["beta_disloc"] * CU["rho_disloc"]
+    assert np.allclose(s, 1 / rho, rtol=1e-14)
+    s2, _ = oracle.cond_layers(dict(CU, rho_disloc=3e15), dict(CU, rho_disloc=3e15), 32, 300.0)
+    assert np.allclose(1 / s2 - 1 / s, CU["beta_disloc"] * 2e15, rtol=1e-9)
+    dT = 120.0
+    s3, _ = oracle.cond_layers(dict(CU, pinterface=1.0), dict(CU, pinterface=1.0), 16, 300.0 + dT)
+    assert np.allclose(s3, CU["condsigma"] / (1 + CU["alpha_resistivity"] * dT), rtol=1e-15)
+
+
+def test_layer_conductivity_grain_boundaries_and_stack():
+    """Grain-boundary term: the Mayadas-Shatzkes ratio f(a) = 1/(1 - 3a/2 +
+    3a^2 - 3a^3 ln(1 + 1/a)) - 1 tends to 3a/2 for small a; the layer stack
+    starts with A, holds round((ny/Nlayers) Vf) rows of B per period, and the
+    bulk difference is a central difference with one-sided ends."""
+    G = dict(CU, pinterface=1.0, rho_disloc=0.0, Ngrains=3, Rgrain=1e-4)
+    s, _ = oracle.cond_layers(G, G, 16, 300.0)
+    a = (G["mfpel"] * 3 / G["dlayer"]) * G["Rgrain"] / (1 - G["Rgrain"])
+    # PS = 0 here: no interface or grain term is added
+    assert np.allclose(s, G["condsigma"], rtol=1e-15)
+    G2 = dict(G, pinterface=0.5)
+    s, _ = oracle.cond_layers(G2, G2, 16, 300.0)
+    rhoi = (3.0 / 16.0) * G2["mfpel"] * (2 * 0.5 / G2["dlayer"]) / G2["condsigma"]
+    small = 1 / (1 / G2["condsigma"] + (2 * rhoi + 2 * (1.5 * a / G2["condsigma"])) / 4)
+    assert abs(s[0] - small) / small < 1e-6
+    sig, dsig = oracle.cond_layers(CU, NB, 128, 300.0)
+    nb_rows = [j for j in range(128) if abs(sig[j] - sig[-1]) < 1e-6 * sig[-1]]
+    Vf = NB["dlayer"] / (CU["dlayer"] + NB["dlayer"])
+    assert len(nb_rows) == math.floor(128 * Vf + 0.5) and nb_rows[0] == 128 - len(nb_rows)
+    bulkA, bulkB = CU["condsigma"], NB["condsigma"]
+    j0 = nb_rows[0]
+    assert dsig[j0 - 1] == pytest.approx((bulkB - bulkA) / 2, rel=1e-15)
+    assert dsig[j0] == pytest.approx((bulkB - bulkA) / 2, rel=1e-15)
+    assert dsig[0] == 0.0 and dsig[127] == 0.0
+    # two periods: 64 * Vf = 6.50 and 64 (1 - Vf) = 57.50 both round up, so
+    # the period is 65 rows, not 64 (the reference's arithmetic as written)
+    sig2, _ = oracle.cond_layers(dict(CU, Nlayers=2), NB, 128, 300.0)
+    dB, dA = math.floor(64 * Vf + 0.5), math.floor(64 * (1 - Vf) + 0.5)
+    assert (dA, dB) == (58, 7)
+    assert [j for j in range(128) if sig2[j] < 1e7] == [j for j in range(128) if j % (dA + dB) >= dA]
+
+
+def test_host_layers_match_oracle():
+    import pfdd_setup
+    for A, B, ny, T in ((CU, NB, 128, 300.0), (dict(CU, Ngrains=4, Nlayers=3), dict(NB, Ngrains=2), 96, 350.0),
+                        (NB, CU, 32, 700.0)):
+        s0, d0 = oracle.cond_layers(A, B, ny, T)
+        s1, d1 = pfdd_setup.cond_layers(A, B, ny, T)
+        assert np.allclose(s1, s0, rtol=1e-15, atol=0)
+        assert np.allclose(d1, d0, rtol=1e-15, atol=1e-300)
+    # the presets' conduction constants
+    cu = pfdd_setup.cond_material("Cu")
+    assert cu["condsigma"] == 5.95e7 and cu["mfpel"] == 39.9e-9 and cu["beta_disloc"] == 1.3e-25
+    nb = pfdd_setup.cond_material("Nb")
+    assert nb["condsigma"] == 6.334e6 and nb["beta_disloc"] == 0.0 and nb["pinterface"] == 0.5
+
+
+def _rand_layers(ny, seed):
+    rng = np.random.default_rng(seed)
+    return 0.5 + rng.random(ny), rng.random(ny) - 0.5
+
+
+def test_cond_step_zero_field_and_plane_wave():
+    """c = 0: j = sigma El everywhere, conductivity = mean sigma, and c moves by
+    rate El dsigma / N.  A plane wave exp(2 pi i m.r/n) transforms to N delta_m,
+    so a = N i f_y/f^2 e, b = N f_y^2/f^2 e after the unnormalised inverse."""
+    shape, four = (8, 16, 8), np.eye(3)
+    sig, dsig = _rand_layers(16, 3)
+    El, eps0, rate = 0.7, 1.3, 0.25
+    c0 = np.zeros(shape, complex)
+    cn, o = oracle.cond_step(c0, four, sig, dsig, El, eps0, rate)
+    N = np.prod(shape)
+    assert o["conductivity"] == pytest.approx(sig.mean(), rel=1e-15)
+    assert o["conductivity_repeat"] == pytest.approx(sig.mean(), rel=1e-15)
+    assert np.allclose(cn, (rate * El * dsig / N)[None, :, None] * np.ones(shape), rtol=1e-15, atol=0)
+    m = (1, 3, 2)
+    i, j, k = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    e = np.exp(2j * np.pi * (m[0] * i / shape[0] + m[1] * j / shape[1] + m[2] * k / shape[2]))
+    fx, fy, fz = (np.array(m, float) / np.array(shape))  # frec on the identity grid, |m| < n/2
+    f2 = fx * fx + fy * fy + fz * fz
+    a, b = N * 1j * fy / f2 * e, N * fy * fy / f2 * e
+    s, ds = sig[None, :, None], dsig[None, :, None]
+    rhs = (s / eps0) * b + (a / eps0 - El) * ds
+    cn, o = oracle.cond_step(e, four, sig, dsig, El, eps0, rate)
+    assert np.max(np.abs(cn - (e - rate * rhs / N))) < 1e-12 * np.max(np.abs(rate * rhs / N))
+    assert o["javerage_re"] == pytest.approx(float(np.mean(s * (El - a.real / eps0))), rel=1e-12)
+
+
+def test_cond_repeat_is_an_index_reflection():
+    """backwardcond run again on its own output transforms a back once more:
+    two unnormalised inverse transforms give N a_hat(-r), so the repeated
+    conductivity is (El mean sigma - sum_k sigma[(-k_y) mod ny] Re a_hat(k) /
+    eps0) / El (the form the GPU evaluates)."""
+    rng = np.random.default_rng(11)
+    shape, four = (8, 8, 16), oracle.set_scale(oracle.prim((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1))[0],
+                                               1, 1, 1)[0]
+    c = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    sig, dsig = _rand_layers(8, 5)
+    El, eps0 = 0.9, 2.0
+    _, o = oracle.cond_step(c, four, sig, dsig, El, eps0, 0.1)
+    ch = oracle.fft3d(c, -1)
+    fx, fy, fz = oracle.frec(*shape, four)
+    f2 = fx * fx + fy * fy + fz * fz
+    are = np.where(f2 > 0, -(ch.imag * fy / np.where(f2 > 0, f2, 1)), 0.0)
+    jy = np.arange(8)
+    srefl = sig[(8 - jy) % 8][None, :, None]
+    want = (El * sig.mean() - np.sum(srefl * are) / eps0) / El
+    assert o["conductivity_repeat"] == pytest.approx(want, rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,skew", [((16, 16, 16), False), ((32, 16, 8), True), ((8, 48, 16), False)])
+def test_gpu_cond_matches_oracle(shape, skew):
+    print(f"case: conductivity step {shape} skew {skew}, 4 iterations, FP64", flush=True)
+    import pfdd_gl
+    rng = np.random.default_rng(7)
+    four = np.eye(3)
+    if skew:
+        four = oracle.set_scale(oracle.prim((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1))[0], 1, 1, 1)[0]
+    c = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    sig, dsig = _rand_layers(shape[1], 9)
+    El, eps0 = 0.7, 1.3
+    rate = 0.05 * np.prod(shape)
+    g = pfdd_gl.CondSolver(shape, four, sig, dsig, El, eps0, rate)
+    try:
+        g.set_c(c)
+        ref = c.copy()
+        for it in range(4):
+            ref, o = oracle.cond_step(ref, four, sig, dsig, El, eps0, rate)
+            r = g.step(1)
+            for key in ("conductivity", "javerage_re", "condxinorm", "condxiave", "conductivity_repeat"):
+                assert abs(r[key] - o[key]) <= 1e-9 * max(abs(o[key]), 1e-300), (it, key, r[key], o[key])
+            assert abs(r["javerage_im"] - o["javerage_im"]) <= 1e-9 * max(abs(o["javerage_re"]), 1.0)
+        got = g.get_c()
+        assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+    finally:
+        g.close()
+
+
+DECK = """seed             576
+app_style        cond100B core_energy 1
+fft_style       hip pbc 1 1 1 mode 1 primitive x 1 0 0 y 0 1 0 z 0 0 1
+dimension        3
+lattice         sc/6n 1
+region          box block 0 {n} 0 {n} 0 {n}
+create_box       box
+create_sites     box
+material       preset Cu dlayer 1.0000000000000001e-07 El 0.0 1e-12 0.0 Rgrain 0.83 Nlayers 1 Ngrains 1 rho_disloc 1e15
+material       preset Nb dlayer 1.1304347826086957e-08 Ngrains 1 rho_disloc 1e15
+solve_style      GLCond max_iter 300 tol 1e-4
+diag_style      strain
+diag_style      stress
+temperature     300.0
+scale           1.0 1.0 1.0
+sigma       initial 0.0 0.0 0.0 0.0 0.06 0.0 delta 0.0 0.0 0.0 0.0 0.0 0.0
+stats           1.0e-8
+run              {run}
+"""
+
+
+def test_conductivity_deck_check_mode(tmp_path):
+    """The reference's conductivity deck parses: app cond100B (one screw system,
+    b = x on n = z, no rotation), two materials, solve_style GLCond."""
+    deck = tmp_path / "in.cond"
+    deck.write_text(DECK.format(n=32, run=2))
+    out = subprocess.run([DRIVER, "-check", "-in", str(deck)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "slip 0 xn 0 0 1 xb 1 0 0" in out.stdout
+    assert "stopsteps 2 max_iter 300" in out.stdout
+
+
+@pytest.mark.gpu
+def test_conductivity_deck_matches_oracle(tmp_path):
+    """in.CuNb-pfdd at 32^3: each stress state runs one iteration (the
+    reference's loop breaks unconditionally) and prints backwardcond's
+    conductivity twice; c starts at zero (FFTW_Slab::allocate)."""
+    print("case: conductivity deck 32^3, run 3 (three stress states)", flush=True)
+    n, nrun = 32, 3
+    deck = tmp_path / "in.cond"
+    deck.write_text(DECK.format(n=n, run=nrun))
+    out = subprocess.run([DRIVER, "-in", str(deck)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    vals = [float(l.split()[1]) for l in out.stdout.splitlines() if l.startswith("conductivity:")]
+    assert len(vals) == 2 * nrun
+    sig, dsig = oracle.cond_layers(CU, NB, n, 300.0)
+    c = np.zeros((n, n, n), complex)
+    for s in range(nrun):
+        c, o = oracle.cond_step(c, np.eye(3), sig, dsig, 1e-12, 8.8541878128e-12, 1e-22)
+        assert vals[2 * s] == pytest.approx(o["conductivity"], rel=1e-12, abs=1e-6)
+        assert vals[2 * s + 1] == pytest.approx(o["conductivity_repeat"], rel=1e-12, abs=1e-6)
