@@ -43,6 +43,38 @@ FFT_HIP::~FFT_HIP()
     if (h_) pfdd_gl_destroy(h_);
 }
 
+void FFT_HIP::project_core_energy()
+{
+    // project_core_energy (fft_fftw_slab.cpp:1550-1624) for plane 0, the one
+    // the apps' compute_delta / compute_ddelta read: delta = Re xi_0 for one or
+    // two systems, else the projection of the plane's three systems onto b_1;
+    // ddelta = forward difference along x on the line j = ny/2, k = nz/2
+    // (dx = 1), zero at the last cell and off the line
+    sync_host();
+    const long long N = (long long)nx * ny * nz;
+    delta.assign(N, 0.0);
+    ddelta.assign(N, 0.0);
+    const std::vector<double> &b = slip.xb;
+    for (long long r = 0; r < N; ++r) {
+        if (slip_systems == 1 || slip_systems == 2) {
+            delta[r] = xi[2 * r];
+        } else {
+            double v = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double p = 0.0;
+                for (int s = 0; s < 3; ++s) p += xi[2 * (r + s * N)] * b[3 * s + c];
+                v += p * b[3 * 1 + c];
+            }
+            delta[r] = v;
+        }
+    }
+    const int j = ny / 2, k = nz / 2;
+    for (int i = 0; i < nx; ++i) {
+        const long long q = (long long)i * ny * nz + (long long)j * nz + k;
+        ddelta[q] = i + 1 == nx ? 0.0 : (delta[q + (long long)ny * nz] - delta[q]) / 1.0;
+    }
+}
+
 void FFT_HIP::cond_init()
 {
     // FFTW_Slab::allocate's condxi (zero) and backwardcond's layer algebra,
@@ -308,6 +340,12 @@ void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
             else if (f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0) {
                 const int a = std::atoi(f.c_str() + 3) - 1;
                 v = fft.xi[2 * (r + (long long)a * N) + (f[2] == 'r' ? 0 : 1)];
+            } else if (f == "deltao" || f == "ddelta") {
+                // App1L1S100B / Appcond100B::compute_delta, compute_ddelta
+                // (app_1L1S_100B.cpp:348-389); other apps return 0 (app.h:132-133)
+                const std::vector<double> &src = f == "deltao" ? fft.delta : fft.ddelta;
+                const bool has = app.style == "1L1S100B" || app.style == "cond100B";
+                v = (has && !src.empty()) ? src[r] : 0.0;
             } else if (f.size() == 3 && (f[0] == 'p' || f[0] == 'e')) {
                 // AppFCC::compute_stress / compute_strain (app_fcc.cpp:438-502): component
                 // (i, j) of the last computed data_sigma / data_eps.  As in the reference,
@@ -424,8 +462,10 @@ void App::init_app(FFT_HIP &fft)
     fft.xi.assign(2 * n, 0.0);
     fft.xo.assign(n, 0.0);
     // the app's default setup_flag when the deck gives no initial command
-    if (!init_set && !fft.slip.default_init.empty()) init_style = fft.slip.default_init;
-    if (init_style == "StraightNonOrtho") {
+    if ((!init_set || fft.slip.force_init) && !fft.slip.default_init.empty()) init_style = fft.slip.default_init;
+    if (init_style == "none") {
+        // initial_sxtal_NoLoop (fft_fftw_slab.cpp:2196-2240): all zero
+    } else if (init_style == "StraightNonOrtho") {
         init_sxtal_straight_nonortho(g, fft.xi.data());
     } else if (init_style == "1LOrthoBCC") {
         init_sxtal_1lorthobcc(g, scale[0], scale[1], fft.xi.data());
@@ -466,9 +506,20 @@ void SolveGL::iterate(App &app, FFT_HIP &fft, Output &out)
     // the first iteration with xinorm < tol, so the break semantics match.
     const double t0 = wall();
     printf("Beginning Time Evolution\n");
+    // init_loop zeroes delta and ddelta at the start of every iteration and
+    // project_core_energy (non_Schmid runs) refills them after the output of
+    // the iteration (solve_GL.cpp:87,119-122): dumps taken during the run see
+    // zeros; the values of the last iteration reach the setup dump of a
+    // following run command
+    bool any = false;
     for (int sstate = 0; sstate < app.stopsteps; ++sstate) {
         fft.rotate_stress();
         fft.resolSS();
+        if (!any) {
+            fft.delta.clear();
+            fft.ddelta.clear();
+            any = true;
+        }
         int it = 0;
         while (it < max_iter) {
             const double need = std::ceil(app.nextoutput - app.time); // first iteration that must output
@@ -497,6 +548,7 @@ void SolveGL::iterate(App &app, FFT_HIP &fft, Output &out)
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) fft.sigma[i][j] = fft.sigma[i][j] + fft.deltasig[i][j];
     }
+    if (any && fft.non_schmid) fft.project_core_energy();
     out.loop_time = wall() - t0;
 }
 
@@ -513,6 +565,8 @@ void SolveGL::iterate_cond(App &app, FFT_HIP &fft, Output &out)
     for (int sstate = 0; sstate < app.stopsteps; ++sstate) {
         fft.rotate_stress();
         fft.resolSS();
+        fft.delta.clear(); // init_loop
+        fft.ddelta.clear();
         for (int it = 0; it < max_iter; ++it) {
             fft.cond_iterate();
             printf("\nconductivity_local: %f \n", fft.cond_last.conductivity);
@@ -523,6 +577,7 @@ void SolveGL::iterate_cond(App &app, FFT_HIP &fft, Output &out)
             if (app.time >= app.nextoutput) app.nextoutput = out.compute(app, fft, app.time, 0);
             app.time += 1;
             app.steps += 1;
+            if (fft.non_schmid) fft.project_core_energy();
             printf("\nconductivity_local: %f \n", fft.cond_last.conductivity_repeat);
             printf("\nconductivity: %f \n", fft.cond_last.conductivity_repeat);
             break;
@@ -577,8 +632,22 @@ void Input::command(const std::vector<std::string> &w)
         SlipTable probe;
         if (!slip_table(app.style, probe)) deck_error("Unrecognized app style " + app.style);
         for (size_t i = 2; i < w.size(); ++i) {
-            if (w[i] == "core_energy") app.core_flag = (int)num(w, ++i, "app_style");
-            else deck_error("Illegal app_style command: " + w[i]);
+            if (w[i] == "core_energy") {
+                app.core_flag = (int)num(w, ++i, "app_style");
+            } else if (w[i] == "setup" && i + 1 < w.size()) {
+                // App::App / set_initial_sxtal (app.cpp:84-88, 810-846): the
+                // setups built here; "screw" is initial_sxtal_3SBCC
+                const std::string st = w[++i];
+                static const char *built[] = {"loop", "none", "edge", "screw", "StraightNonOrtho", "1LOrthoBCC",
+                                              "frank_read"};
+                bool ok = false;
+                for (auto b : built) ok |= st == b;
+                if (!ok) deck_error("setup " + st + " is not built in this driver (use the initial command)");
+                app.init_style = st == "screw" ? "3SBCC" : st;
+                app.init_set = true;
+            } else {
+                deck_error("Illegal app_style command: " + w[i]);
+            }
         }
     } else if (c == "fft_style") {
         // FFT::FFT argument parsing (fft.cpp:96-153); style hip (or the
@@ -746,8 +815,10 @@ void Input::command(const std::vector<std::string> &w)
                                            "exx", "eyy", "ezz", "exy", "exz", "eyz"};
             bool t = false;
             for (auto q : tens) t |= f == q;
-            const bool ok = t || f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0;
-            if (!ok) deck_error("dump field " + f + " is not available (x y z xirN xiiN pxx..pyz exx..eyz)");
+            const bool ok = t || f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 ||
+                            f.rfind("xii", 0) == 0 || f == "deltao" || f == "ddelta";
+            if (!ok)
+                deck_error("dump field " + f + " is not available (x y z xirN xiiN pxx..pyz exx..eyz deltao ddelta)");
             if (t) fft.want_fields = true;
             d.fields.push_back(f);
         }

@@ -81,6 +81,10 @@ class FFT_HIP {
     // conductivity sub-problem of solve_style GLCond: the second material
     // command (fft->materialB), the temperature command, the per-row layer
     // conductivities, and the GPU field c (fft->condxi)
+    // fft->delta / ddelta of plane 0 (project_core_energy, fft_fftw_slab.cpp:
+    // 1550-1624), the arrays the dump fields deltao / ddelta read
+    std::vector<double> delta, ddelta;
+    void project_core_energy();
     Material matB;
     bool have_matB = false, cond = false;
     double temperature = 300.0;

@@ -293,8 +293,11 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.xb = {0.5, 0.5, 0.5};
         return true;
     }
-    if (app == "frank_read") {
-        // AppFrankRead (app_frank_read.cpp:29-50, 96-120)
+    if (app == "frank_read" || app == "FrankRead") {
+        // AppFrankRead (app_frank_read.cpp:29-50, 96-120); its init_app always
+        // sets up initial_sxtal_frank_read (:77-85, 127-130)
+        t.default_init = "frank_read";
+        t.force_init = app == "FrankRead";
         t.slip_systems = 2;
         t.num_planes = 2;
         t.CD = 0.35;

@@ -58,11 +58,12 @@ struct SlipTable {
     double timestep = 1.0; // app->timestep
     int default_core = 1;
     std::string default_init; // the app's default setup_flag as an initial style ("" = none built)
+    bool force_init = false;  // the app's init_app ignores the deck's setup (AppFrankRead)
     bool rotate = true;         // set_slip applies the four/normfour rotation
     std::vector<double> xn, xb; // S*3 each
 };
 
-// Known app styles: fcc, 3d-hcp-basal, frank_read, 1L1S100B, cond100B,
+// Known app styles: fcc, 3d-hcp-basal, FrankRead (alias frank_read), 1L1S100B, cond100B,
 // 2d-bcc-1lOrtho, and basal3 (three unit b at 0/120/240 deg on the plane
 // n = z).  Returns false for unknown names.
 bool slip_table(const std::string &app, SlipTable &t, double oflag = 0.0);

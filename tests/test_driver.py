@@ -300,3 +300,33 @@ def test_deck_non_schmid_usfe_run_matches_oracle(tmp_path):
         sigma = sigma + dsig
     err = np.max(np.abs(xi - st.xi)) / np.max(np.abs(st.xi))
     assert err < 1e-9, err
+
+
+@pytest.mark.gpu
+def test_deltao_ddelta_dumps_as_executed(tmp_path):
+    """project_core_energy (fft_fftw_slab.cpp:1550-1624) runs after the output
+    of each iteration of a non_Schmid run and init_loop zeroes delta / ddelta
+    at the start of the next, so dumps inside a run read zeros; the setup
+    snapshot of a second run command reads the last projection: for the one
+    system of 1L1S100B, deltao = Re xi_1 and ddelta the forward x difference
+    on the line j = ny/2, k = nz/2."""
+    print("case: driver deck 1L1S100B 16^3, non_Schmid, deltao/ddelta dumps over two runs", flush=True)
+    text = deck_text(app="1L1S100B", max_iter=1, tol=0, run=3, s0="0 0 0 0 0.002 0", noise=1e-2,
+                     extra="non_Schmid 0.0 0.1 0.2 0.3\ndump 1 text 1 d.txt x y z xir1 deltao ddelta\n")
+    text = text.replace("app_style 1L1S100B core_energy 3", "app_style 1L1S100B core_energy 1")
+    text = text.replace("run 3\n", "run 3\nrun 2\n")
+    r = run_exe(text, tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    snaps = read_text_dumps(tmp_path / "d.txt")
+    # run 1: setup + 3 states; run 2: setup + 2 states
+    assert len(snaps) == 7
+    for q, (_, _, cols, rows) in enumerate(snaps):
+        assert cols == ["x", "y", "z", "xir1", "deltao", "ddelta"]
+        if q != 4:
+            assert np.all(rows[:, 4] == 0) and np.all(rows[:, 5] == 0), q
+    rows = snaps[4][3]
+    assert np.any(rows[:, 3] != 0) and np.allclose(rows[:, 4], rows[:, 3], rtol=1e-5, atol=1e-6)
+    x = rows[:, 3].reshape(16, 16, 16)
+    want = np.zeros((16, 16, 16))
+    want[:-1, 8, 8] = x[1:, 8, 8] - x[:-1, 8, 8]
+    assert np.allclose(rows[:, 5].reshape(16, 16, 16), want, rtol=1e-4, atol=1e-5)
