@@ -318,8 +318,9 @@ def test_deltao_ddelta_dumps_as_executed(tmp_path):
     r = run_exe(text, tmp_path)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     snaps = read_text_dumps(tmp_path / "d.txt")
-    # run 1: setup + 3 states; run 2: setup + 2 states
-    assert len(snaps) == 7
+    # run 1: setup + 3 states; run 2 (stoptime 3 + 2 = 5, and App::init_app
+    # recomputes stopsteps = floor(stoptime / timestep) = 5): setup + 5 states
+    assert len(snaps) == 10
     for q, (_, _, cols, rows) in enumerate(snaps):
         assert cols == ["x", "y", "z", "xir1", "deltao", "ddelta"]
         if q != 4:
