@@ -271,9 +271,14 @@ __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__res
 }
 
 // --------------------------------------------- pass 3: x FFT + contraction
+// Columns per x-pass tile: the largest power of two whose S*NP lines of N
+// elements stay within `target` elements.  FP64 with 12 systems takes twice
+// the tile of the other FP64 cases: at 128^3 x 12 two columns per block (one
+// thread per k-point, no duplicated contraction work, 3 blocks per CU) ran
+// the pass 12 % faster than one column (two threads per k-point, 5 blocks).
 template <typename R, int N, int S> constexpr int mid_pairs()
 {
-    constexpr int target = sizeof(R) == 8 ? 2048 : 4096;
+    constexpr int target = sizeof(R) == 8 ? (S == 12 ? 4096 : 2048) : 4096;
     int np = 1;
     while (np * 2 * S * N <= target) np *= 2;
     return np;
