@@ -463,8 +463,14 @@ void App::init_app(FFT_HIP &fft)
     fft.xo.assign(n, 0.0);
     // the app's default setup_flag when the deck gives no initial command
     if ((!init_set || fft.slip.force_init) && !fft.slip.default_init.empty()) init_style = fft.slip.default_init;
+    if (!init_set && fft.slip.default_init.empty() && !fft.slip.missing_init.empty())
+        printf("WARNING: the default setup %s of app %s is not built here; the initial field is zero "
+               "(give an initial command or app_style ... setup)\n",
+               fft.slip.missing_init.c_str(), style.c_str());
     if (init_style == "none") {
         // initial_sxtal_NoLoop (fft_fftw_slab.cpp:2196-2240): all zero
+    } else if (init_style == "StraightOrtho") {
+        init_sxtal_straight_ortho(g, fft.xi.data());
     } else if (init_style == "StraightNonOrtho") {
         init_sxtal_straight_nonortho(g, fft.xi.data());
     } else if (init_style == "1LOrthoBCC") {
@@ -638,8 +644,8 @@ void Input::command(const std::vector<std::string> &w)
                 // App::App / set_initial_sxtal (app.cpp:84-88, 810-846): the
                 // setups built here; "screw" is initial_sxtal_3SBCC
                 const std::string st = w[++i];
-                static const char *built[] = {"loop", "none", "edge", "screw", "StraightNonOrtho", "1LOrthoBCC",
-                                              "frank_read"};
+                static const char *built[] = {"loop", "none", "edge", "screw", "StraightOrtho", "StraightNonOrtho",
+                                              "1LOrthoBCC", "frank_read"};
                 bool ok = false;
                 for (auto b : built) ok |= st == b;
                 if (!ok) deck_error("setup " + st + " is not built in this driver (use the initial command)");

@@ -232,6 +232,7 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.CD = 0.5;
         t.timestep = 1.0;
         t.default_core = 1;
+        t.missing_init = "mode1";
         for (int a = 0; a < 12; ++a)
             for (int c = 0; c < 3; ++c) {
                 t.xn.push_back(sg[a / 3][c] / s3);
@@ -245,6 +246,7 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.num_planes = 3;
         t.CD = 0.5;
         t.default_core = 1;
+        t.missing_init = "3D3ShcpBasal";
         const double ang[3] = {oflag * M_PI / 2.0, oflag * M_PI / 2.0 + M_PI / 3.0,
                                oflag * M_PI / 2.0 - M_PI / 3.0};
         for (int a = 0; a < 3; ++a) {
@@ -292,6 +294,76 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.xn = {1.0 / std::sqrt(2), -1.0 / std::sqrt(2), 0.0};
         t.xb = {0.5, 0.5, 0.5};
         return true;
+    }
+    {
+        // The single-table apps of the reference: slip systems, mobility,
+        // whether set_slip rotates by four/normfour, and the default setup
+        // (setup_flag of the constructor; "" = that initial configuration is
+        // not built here).  Vectors as set_slip writes them.
+        const double r2 = 1.0 / std::sqrt(2.0), r3 = 1.0 / std::sqrt(3.0);
+        struct AppRow {
+            const char *name;
+            int S;
+            double CD;
+            bool rotate;
+            const char *init, *missing; // built default setup / name of an unbuilt one
+            double xn[9], xb[9];
+        };
+        const AppRow rows[] = {
+            // app_1L1S111B_bcc.cpp:29-49, 212-228 (rotation block commented out)
+            {"1L1S111Bbcc", 1, 0.5, false, "1LOrthoBCC", "", {0, 0, 1}, {1, 0, 0}},
+            // app_1L1S_110B_fcc.cpp, _fcc_nonortho.cpp, _inclined.cpp, app_1L1S_111B.cpp
+            {"1L1S110Bfcc", 1, 0.5, false, "", "1LOrthoFCC", {r3, r3, r3}, {-r2, r2, 0}},
+            {"1L1S110Bfccnonortho", 1, 0.5, false, "", "1LNonOrthoFCC", {r3, r3, r3}, {r2, -r2, 0}},
+            {"1L1S110Binclined", 1, 0.5, false, "", "1LInclinedFCC", {r3, r3, r3}, {r2, -r2, 0}},
+            {"1L1S111B", 1, 0.5, false, "", "1LOrthoFCC", {r3, r3, r3}, {-0.5, 0.5, 0}},
+            // app_1SBCC.cpp, app_3SBCC.cpp, app_bcc_edge.cpp, app_bcc_edge_planar.cpp
+            {"1SBCC", 1, 0.25, false, "3SBCC", "", {0.7071, 0.7071, 0}, {-0.57735, 0.57735, 0.57735}},
+            {"3SBCC", 3, 0.25, false, "3SBCC", "", {0.7071, 0.7071, 0, 0, 0.7071, -0.7071, 0.7071, 0, 0.7071},
+             {-0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735}},
+            {"BCCEdge", 3, 0.25, false, "edge", "", {0.7071, 0.7071, 0, 0, 0.7071, -0.7071, 0.7071, 0, 0.7071},
+             {-0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735}},
+            {"BCCEdgePlanar", 1, 0.25, false, "edge", "", {0.7071, 0.7071, 0}, {-0.57735, 0.57735, 0.57735}},
+            // app_2L2S_110B.cpp (no rotation), app_2d.cpp (rotated)
+            {"2L2S110B", 2, 0.5, false, "", "mode1", {r3, r3, r3, r3, r3, -r3}, {r2, -r2, 0, r2, -r2, 0}},
+            {"2d", 1, 0.5, true, "", "1LOrthoFCC", {r3, r3, r3}, {0.5, -0.5, 0}},
+            // app_3d_2slip*.cpp (rotated)
+            {"3d-2slip", 1, 0.5, true, "1LOrthoBCC", "", {r3, r3, r3}, {-0.5, 0.5, 0}},
+            {"3d2s2lOrtho", 2, 0.5, true, "1LOrthoBCC", "", {r3, r3, r3, r3, r3, -r3}, {0.5, -0.5, 0, 0.5, -0.5, 0}},
+            {"3d2sStraightNonOrtho", 2, 0.5, true, "StraightNonOrtho", "", {-r3, r3, -r3, r3, -r3, -r3},
+             {0.5, 0.5, 0, 0.5, 0.5, 0}},
+            {"3d2sStraightOrtho", 2, 0.5, true, "StraightOrtho", "", {r3, r3, r3, r3, r3, -r3},
+             {0.5, -0.5, 0, 0.5, -0.5, 0}},
+        };
+        for (const AppRow &r : rows) {
+            if (app != r.name) continue;
+            t.slip_systems = r.S;
+            t.num_planes = r.S;
+            t.CD = r.CD;
+            t.default_core = 1;
+            t.rotate = r.rotate;
+            t.default_init = r.init;
+            t.missing_init = r.missing;
+            t.xn.assign(r.xn, r.xn + 3 * r.S);
+            t.xb.assign(r.xb, r.xb + 3 * r.S);
+            return true;
+        }
+        if (app == "3d3sHCPnotch") {
+            // App3d3sHCPnotch (app_3d3s_hcp_notch.cpp:29-50, set_slip): the
+            // 3d-hcp-basal vectors with oflag = 0, rotated; default setup
+            // 3D3ShcpNotch
+            t.slip_systems = 3;
+            t.num_planes = 3;
+            t.CD = 0.5;
+            t.default_core = 1;
+            t.missing_init = "3D3ShcpNotch";
+            const double ang[3] = {0.0, M_PI / 3.0, -M_PI / 3.0};
+            for (int a = 0; a < 3; ++a) {
+                t.xn.insert(t.xn.end(), {0.0, 0.0, 1.0});
+                t.xb.insert(t.xb.end(), {1.0 * std::cos(ang[a]), 1.0 * std::sin(ang[a]), 0.0});
+            }
+            return true;
+        }
     }
     if (app == "frank_read" || app == "FrankRead") {
         // AppFrankRead (app_frank_read.cpp:29-50, 96-120); its init_app always
@@ -630,6 +702,24 @@ void init_sxtal_straight_nonortho(const Grid &g, double *xi)
                     const double dk = k - g.nz / 2.0;
                     const int d = 2 * i - j;
                     const bool on = dk < eps && dk > -eps && (d <= -g.nx || (d >= 0 && d <= g.nx)) && is == 0;
+                    xi[2 * idx] = on ? 1.0 : 0.0;
+                    xi[2 * idx + 1] = 0.0;
+                }
+}
+
+void init_sxtal_straight_ortho(const Grid &g, double *xi)
+{
+    // initial_sxtal_StraightOrtho (fft_fftw_slab.cpp:2745-2800): system 0,
+    // the plane |k - nz/2| < 0.2, rows ny/4 < j < 3 ny/4
+    const long long N = g.N();
+    const double eps = 2e-1;
+    for (int is = 0; is < g.S; ++is)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    const long long idx = k + (long long)j * g.nz + (long long)i * g.ny * g.nz + (long long)is * N;
+                    const double dk = k - g.nz / 2.0;
+                    const bool on = dk < eps && dk > -eps && j > g.ny / 4.0 && j < 3.0 * g.ny / 4.0 && is == 0;
                     xi[2 * idx] = on ? 1.0 : 0.0;
                     xi[2 * idx + 1] = 0.0;
                 }

@@ -59,13 +59,17 @@ struct SlipTable {
     int default_core = 1;
     std::string default_init; // the app's default setup_flag as an initial style ("" = none built)
     bool force_init = false;  // the app's init_app ignores the deck's setup (AppFrankRead)
+    std::string missing_init; // the app's default setup when it is not built here
     bool rotate = true;         // set_slip applies the four/normfour rotation
     std::vector<double> xn, xb; // S*3 each
 };
 
-// Known app styles: fcc, 3d-hcp-basal, FrankRead (alias frank_read), 1L1S100B, cond100B,
-// 2d-bcc-1lOrtho, and basal3 (three unit b at 0/120/240 deg on the plane
-// n = z).  Returns false for unknown names.
+// Known app styles: every AppStyle of the reference (fcc, 3d-hcp-basal,
+// FrankRead (alias frank_read), 1L1S100B, cond100B, 2d-bcc-1lOrtho,
+// 1L1S111Bbcc, 1L1S110Bfcc, 1L1S110Bfccnonortho, 1L1S110Binclined, 1L1S111B,
+// 1SBCC, 3SBCC, BCCEdge, BCCEdgePlanar, 2L2S110B, 2d, 3d-2slip, 3d2s2lOrtho,
+// 3d2sStraightNonOrtho, 3d2sStraightOrtho, 3d3sHCPnotch) and basal3 (three
+// unit b at 0/120/240 deg on the plane n = z).  Returns false otherwise.
 bool slip_table(const std::string &app, SlipTable &t, double oflag = 0.0);
 
 // AppFCC::set_slip rotation block (app_fcc.cpp:288-313).
@@ -125,6 +129,8 @@ void init_sxtal_frank_read(const Grid &g, double *xi, double *xo);
 void init_sxtal_edge(const Grid &g, double *xi);
 // initial_sxtal_StraightNonOrtho (fft_fftw_slab.cpp:2804-2855).
 void init_sxtal_straight_nonortho(const Grid &g, double *xi);
+// initial_sxtal_StraightOrtho (fft_fftw_slab.cpp:2745-2800).
+void init_sxtal_straight_ortho(const Grid &g, double *xi);
 // initial_sxtal_1LOrthoBCC (fft_fftw_slab.cpp:2600-2660); sa, sb = scale command.
 void init_sxtal_1lorthobcc(const Grid &g, double sa, double sb, double *xi);
 // initial_sxtal_3SBCC (fft_fftw_slab.cpp:3095-3144).

@@ -166,6 +166,7 @@ int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, doub
     else if (s == "edge") init_sxtal_edge(g, xi);
     else if (s == "3SBCC") init_sxtal_3sbcc(g, xi);
     else if (s == "StraightNonOrtho") init_sxtal_straight_nonortho(g, xi);
+    else if (s == "StraightOrtho") init_sxtal_straight_ortho(g, xi);
     else if (s == "1LOrthoBCC") init_sxtal_1lorthobcc(g, 1.0, 1.0, xi);
     else return 1;
     if (xo && s != "frank_read")

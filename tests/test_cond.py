@@ -227,3 +227,15 @@ def test_conductivity_deck_matches_oracle(tmp_path):
         c, o = oracle.cond_step(c, np.eye(3), sig, dsig, 1e-12, 8.8541878128e-12, 1e-22)
         assert vals[2 * s] == pytest.approx(o["conductivity"], rel=1e-12, abs=1e-6)
         assert vals[2 * s + 1] == pytest.approx(o["conductivity_repeat"], rel=1e-12, abs=1e-6)
+
+
+def test_cond_create_rejects_bad_descriptors():
+    """Argument checks of pfdd_gl_cond_create run before any HIP call."""
+    import pfdd_gl
+    sig, dsig = _rand_layers(16, 1)
+    with pytest.raises(pfdd_gl.PfddError) as ei:
+        pfdd_gl.CondSolver((16, 16, 16), np.eye(3), sig, dsig, El=0.0)
+    assert ei.value.code == 1
+    with pytest.raises(pfdd_gl.PfddError) as ei:
+        pfdd_gl.CondSolver((10, 16, 16), np.eye(3), sig, dsig, El=1e-12)
+    assert ei.value.code == 2

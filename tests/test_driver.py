@@ -331,3 +331,50 @@ def test_deltao_ddelta_dumps_as_executed(tmp_path):
     want = np.zeros((16, 16, 16))
     want[:-1, 8, 8] = x[1:, 8, 8] - x[:-1, 8, 8]
     assert np.allclose(rows[:, 5].reshape(16, 16, 16), want, rtol=1e-4, atol=1e-5)
+
+
+APP_STYLES = {  # AppStyle key -> (slip systems, set_slip rotates by four/normfour)
+    "fcc": (12, True), "3d-hcp-basal": (3, True), "FrankRead": (2, False), "1L1S100B": (1, False),
+    "cond100B": (1, False), "2d-bcc-1lOrtho": (1, True), "1L1S111Bbcc": (1, False), "1L1S110Bfcc": (1, False),
+    "1L1S110Bfccnonortho": (1, False), "1L1S110Binclined": (1, False), "1L1S111B": (1, False),
+    "1SBCC": (1, False), "3SBCC": (3, False), "BCCEdge": (3, False), "BCCEdgePlanar": (1, False),
+    "2L2S110B": (2, False), "2d": (1, True), "3d-2slip": (1, True), "3d2s2lOrtho": (2, True),
+    "3d2sStraightNonOrtho": (2, True), "3d2sStraightOrtho": (2, True), "3d3sHCPnotch": (3, True),
+}
+
+
+def _slips(out):
+    xn, xb = [], []
+    for line in out.splitlines():
+        w = line.split()
+        if w and w[0] == "slip":
+            xn.append([float(v) for v in w[3:6]])
+            xb.append([float(v) for v in w[7:10]])
+    return np.array(xn), np.array(xb)
+
+
+@pytestmark_cpu
+@pytest.mark.parametrize("app", sorted(APP_STYLES))
+def test_every_reference_app_style_parses(tmp_path, app):
+    """Every AppStyle key of the reference is accepted: its slip systems are
+    unit plane normals perpendicular to their Burgers vectors on the cubic
+    grid, and on a skewed grid they are rotated by four/normfour exactly when
+    the app's set_slip rotates (checked with the oracle's rotate_slip)."""
+    S, rot = APP_STYLES[app]
+    base = deck_text(app=app).replace(f"app_style {app} core_energy 3", f"app_style {app} core_energy 1")
+    r = run_exe(base, tmp_path, check=True)
+    assert r.returncode == 0, r.stdout[-1500:] + r.stderr[-1500:]
+    xn, xb = _slips(r.stdout)
+    assert xn.shape == (S, 3)
+    assert np.allclose(np.linalg.norm(xn, axis=1), 1.0, atol=2e-4)
+    assert np.allclose(np.sum(xn * xb, axis=1), 0.0, atol=2e-4)
+    skew = base.replace("primitive x 1 0 0 y 0 1 0 z 0 0 1", "primitive x 1 0 0 y 0.5 0.8660254 0 z 0 0 1")
+    r2 = run_exe(skew, tmp_path, check=True)
+    assert r2.returncode == 0
+    xn2, xb2 = _slips(r2.stdout)
+    if rot:
+        geo = P.geometry(((1, 0, 0), (0.5, 0.8660254, 0), (0, 0, 1)))
+        wn, wb = oracle.rotate_slip(xn, xb, geo["four"], geo["normfour"])
+        assert np.allclose(xn2, wn, rtol=1e-12, atol=1e-15) and np.allclose(xb2, wb, rtol=1e-12, atol=1e-15)
+    else:
+        assert np.array_equal(xn2, xn) and np.array_equal(xb2, xb)
