@@ -64,7 +64,10 @@ int pfdd_host_init_random_loops(int nx, int ny, int nz, const char *app, int nlo
 /* The reference's hard-coded initial conditions: style "loop"
  * (initial_sxtal_loop :3146-3197), "frank_read" (:3032-3093, also fills the
  * obstacle mask xo if non-NULL), "edge" (:3201-3247), "3SBCC" (:3095-3144),
- * "StraightNonOrtho" (:2804-2855), "1LOrthoBCC" (:2600-2660, scale 1 1). */
+ * "StraightNonOrtho" (:2804-2855), "StraightOrtho" (:2745-2800), "1LOrthoBCC"
+ * (:2600-2660), "mode1", "mode2", "1LOrthoFCC", "1LNonOrthoFCC",
+ * "1LInclinedFCC", "2LOrtho", "3D3ShcpBasal", "3D3ShcpNotch" (:2245-2742,
+ * 2864-3029; the notch also fills xo) -- on the cubic grid, scale 1 1 1. */
 int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, double *xi, double *xo);
 
 /* Conduction constants of a material (material.h:44-56 defaults, then the

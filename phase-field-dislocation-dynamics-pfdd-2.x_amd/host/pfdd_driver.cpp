@@ -469,6 +469,8 @@ void App::init_app(FFT_HIP &fft)
                fft.slip.missing_init.c_str(), style.c_str());
     if (init_style == "none") {
         // initial_sxtal_NoLoop (fft_fftw_slab.cpp:2196-2240): all zero
+    } else if (init_sxtal_more(init_style, g, fft.geo, scale[0], (double)fft.nx, fft.xi.data(), fft.xo.data())) {
+        // mode1, mode2, 1L*FCC, 2LOrtho, 3D3Shcp*
     } else if (init_style == "StraightOrtho") {
         init_sxtal_straight_ortho(g, fft.xi.data());
     } else if (init_style == "StraightNonOrtho") {
@@ -644,8 +646,10 @@ void Input::command(const std::vector<std::string> &w)
                 // App::App / set_initial_sxtal (app.cpp:84-88, 810-846): the
                 // setups built here; "screw" is initial_sxtal_3SBCC
                 const std::string st = w[++i];
-                static const char *built[] = {"loop", "none", "edge", "screw", "StraightOrtho", "StraightNonOrtho",
-                                              "1LOrthoBCC", "frank_read"};
+                static const char *built[] = {"loop",          "none",          "edge",          "screw",
+                                              "mode1",         "mode2",         "1LOrthoFCC",    "1LNonOrthoFCC",
+                                              "1LInclinedFCC", "1LOrthoBCC",    "2LOrtho",       "StraightOrtho",
+                                              "StraightNonOrtho", "frank_read", "3D3ShcpNotch",  "3D3ShcpBasal"};
                 bool ok = false;
                 for (auto b : built) ok |= st == b;
                 if (!ok) deck_error("setup " + st + " is not built in this driver (use the initial command)");

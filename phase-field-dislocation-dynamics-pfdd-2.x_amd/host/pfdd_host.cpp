@@ -232,7 +232,7 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.CD = 0.5;
         t.timestep = 1.0;
         t.default_core = 1;
-        t.missing_init = "mode1";
+        t.default_init = "mode1";
         for (int a = 0; a < 12; ++a)
             for (int c = 0; c < 3; ++c) {
                 t.xn.push_back(sg[a / 3][c] / s3);
@@ -246,7 +246,7 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
         t.num_planes = 3;
         t.CD = 0.5;
         t.default_core = 1;
-        t.missing_init = "3D3ShcpBasal";
+        t.default_init = "3D3ShcpBasal";
         const double ang[3] = {oflag * M_PI / 2.0, oflag * M_PI / 2.0 + M_PI / 3.0,
                                oflag * M_PI / 2.0 - M_PI / 3.0};
         for (int a = 0; a < 3; ++a) {
@@ -313,10 +313,10 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
             // app_1L1S111B_bcc.cpp:29-49, 212-228 (rotation block commented out)
             {"1L1S111Bbcc", 1, 0.5, false, "1LOrthoBCC", "", {0, 0, 1}, {1, 0, 0}},
             // app_1L1S_110B_fcc.cpp, _fcc_nonortho.cpp, _inclined.cpp, app_1L1S_111B.cpp
-            {"1L1S110Bfcc", 1, 0.5, false, "", "1LOrthoFCC", {r3, r3, r3}, {-r2, r2, 0}},
-            {"1L1S110Bfccnonortho", 1, 0.5, false, "", "1LNonOrthoFCC", {r3, r3, r3}, {r2, -r2, 0}},
-            {"1L1S110Binclined", 1, 0.5, false, "", "1LInclinedFCC", {r3, r3, r3}, {r2, -r2, 0}},
-            {"1L1S111B", 1, 0.5, false, "", "1LOrthoFCC", {r3, r3, r3}, {-0.5, 0.5, 0}},
+            {"1L1S110Bfcc", 1, 0.5, false, "1LOrthoFCC", "", {r3, r3, r3}, {-r2, r2, 0}},
+            {"1L1S110Bfccnonortho", 1, 0.5, false, "1LNonOrthoFCC", "", {r3, r3, r3}, {r2, -r2, 0}},
+            {"1L1S110Binclined", 1, 0.5, false, "1LInclinedFCC", "", {r3, r3, r3}, {r2, -r2, 0}},
+            {"1L1S111B", 1, 0.5, false, "1LOrthoFCC", "", {r3, r3, r3}, {-0.5, 0.5, 0}},
             // app_1SBCC.cpp, app_3SBCC.cpp, app_bcc_edge.cpp, app_bcc_edge_planar.cpp
             {"1SBCC", 1, 0.25, false, "3SBCC", "", {0.7071, 0.7071, 0}, {-0.57735, 0.57735, 0.57735}},
             {"3SBCC", 3, 0.25, false, "3SBCC", "", {0.7071, 0.7071, 0, 0, 0.7071, -0.7071, 0.7071, 0, 0.7071},
@@ -325,8 +325,8 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
              {-0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735, -0.57735, 0.57735, 0.57735}},
             {"BCCEdgePlanar", 1, 0.25, false, "edge", "", {0.7071, 0.7071, 0}, {-0.57735, 0.57735, 0.57735}},
             // app_2L2S_110B.cpp (no rotation), app_2d.cpp (rotated)
-            {"2L2S110B", 2, 0.5, false, "", "mode1", {r3, r3, r3, r3, r3, -r3}, {r2, -r2, 0, r2, -r2, 0}},
-            {"2d", 1, 0.5, true, "", "1LOrthoFCC", {r3, r3, r3}, {0.5, -0.5, 0}},
+            {"2L2S110B", 2, 0.5, false, "mode1", "", {r3, r3, r3, r3, r3, -r3}, {r2, -r2, 0, r2, -r2, 0}},
+            {"2d", 1, 0.5, true, "1LOrthoFCC", "", {r3, r3, r3}, {0.5, -0.5, 0}},
             // app_3d_2slip*.cpp (rotated)
             {"3d-2slip", 1, 0.5, true, "1LOrthoBCC", "", {r3, r3, r3}, {-0.5, 0.5, 0}},
             {"3d2s2lOrtho", 2, 0.5, true, "1LOrthoBCC", "", {r3, r3, r3, r3, r3, -r3}, {0.5, -0.5, 0, 0.5, -0.5, 0}},
@@ -356,7 +356,7 @@ bool slip_table(const std::string &app, SlipTable &t, double oflag)
             t.num_planes = 3;
             t.CD = 0.5;
             t.default_core = 1;
-            t.missing_init = "3D3ShcpNotch";
+            t.default_init = "3D3ShcpNotch";
             const double ang[3] = {0.0, M_PI / 3.0, -M_PI / 3.0};
             for (int a = 0; a < 3; ++a) {
                 t.xn.insert(t.xn.end(), {0.0, 0.0, 1.0});
@@ -705,6 +705,115 @@ void init_sxtal_straight_nonortho(const Grid &g, double *xi)
                     xi[2 * idx] = on ? 1.0 : 0.0;
                     xi[2 * idx + 1] = 0.0;
                 }
+}
+
+bool init_sxtal_more(const std::string &style, const Grid &g, const Geometry &geo, double sa, double xprd, double *xi,
+                     double *xo)
+{
+    // The remaining hard-coded configurations of FFTW_Slab (fft_fftw_slab.cpp):
+    // mode1 :2245-2333, mode2 :2336-2410, 1LOrthoFCC :2413-2471,
+    // 1LNonOrthoFCC :2474-2537, 1LInclinedFCC :2540-2597, 2LOrtho :2665-2742,
+    // 3D3ShcpBasal :2864-2946, 3D3ShcpNotch :2949-3029 (also the mask xo).
+    // Cell (i, j, k) of system is; the plane tests |.| < eps are the
+    // reference's; sclprim maps grid offsets to lengths (mode1, 1LNonOrthoFCC).
+    const long long N = g.N();
+    const int N1 = g.nx, N2 = g.ny, N3 = g.nz;
+    auto len = [&](double a, double b, double c) {
+        double r2 = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            const double v = geo.sclprim[q][0] * a + geo.sclprim[q][1] * b + geo.sclprim[q][2] * c;
+            r2 += v * v;
+        }
+        return std::sqrt(r2);
+    };
+    int which;
+    if (style == "mode1") which = 0;
+    else if (style == "mode2") which = 1;
+    else if (style == "1LOrthoFCC") which = 2;
+    else if (style == "1LNonOrthoFCC") which = 3;
+    else if (style == "1LInclinedFCC") which = 4;
+    else if (style == "2LOrtho") which = 5;
+    else if (style == "3D3ShcpBasal") which = 6;
+    else if (style == "3D3ShcpNotch") which = 7;
+    else return false;
+    for (int is = 0; is < g.S; ++is)
+        for (int i = 0; i < N1; ++i)
+            for (int j = 0; j < N2; ++j)
+                for (int k = 0; k < N3; ++k) {
+                    const long long idx = k + (long long)j * N3 + (long long)i * N2 * N3 + (long long)is * N;
+                    bool on = false, mask = false;
+                    const double dk2 = k - N3 / 2.0;
+                    switch (which) {
+                    case 0: {
+                        const double eps = 1e-1, dj2 = j - N2 / 2.0;
+                        if (dk2 < eps && dk2 > -eps && is == 0)
+                            on = len(i - N1 / 2.0, j - 5.0 * N2 / 8.0, k - N3 / 2.0) <= N2 / 8.0 - 2.0;
+                        else if (dj2 < eps && dj2 > -eps && is == 1)
+                            on = len(i - N1 / 2.0 + N1 / 8.0, j - N2 / 2.0, k - 5.0 * N3 / 8.0) <= N3 / 8.0 - 2.0;
+                        break;
+                    }
+                    case 1: {
+                        const double eps = 1e-1, p1 = i + j + k - N1, p2 = i + j - k;
+                        if (p1 < eps && p1 > -eps && is == 0)
+                            on = sa * std::sqrt((i - 28.0) * (i - 28.0) + (j - 28.0) * (j - 28.0) + (k - 40.0) * (k - 40.0)) <=
+                                 3.0 * std::sqrt(2.0);
+                        else if (p2 < eps && p2 > -eps && is == 1)
+                            on = sa * std::sqrt((i - 28.0) * (i - 28.0) + (j - 28.0) * (j - 28.0) + (k - 56.0) * (k - 56.0)) <=
+                                 3.0 * std::sqrt(2.0);
+                        break;
+                    }
+                    case 2: {
+                        const double eps = 1e-1;
+                        if (dk2 < eps && dk2 > -eps && is == 0)
+                            on = std::sqrt((i - N1 / 2.0) * (i - N1 / 2.0) + (j - N2 / 2.0) * (j - N2 / 2.0)) <= xprd / 10.0;
+                        break;
+                    }
+                    case 3: {
+                        const double eps = 1e-1;
+                        if (dk2 < eps && dk2 > -eps && is == 0)
+                            on = len(i - N1 / 2.0, j - N2 / 2.0, k - N3 / 2.0) <= 10.0;
+                        break;
+                    }
+                    case 4: {
+                        const double eps = 1e-1, p1 = i + j + k - N1;
+                        if (p1 < eps && p1 > -eps && is == 0)
+                            on = sa * std::sqrt((i - N1 / 3.0) * (i - N1 / 3.0) + (j - N2 / 3.0) * (j - N2 / 3.0) +
+                                                (k - N3 / 3.0) * (k - N3 / 3.0)) <= 5.0 * std::sqrt(2.0);
+                        break;
+                    }
+                    case 5: {
+                        const double eps = 1e-1, t = 2.8284 * (j - N2 / 2.0) + (k - N3 / 2.0), dk4 = k - N3 / 4.0;
+                        if (t < 20.0 * eps && t > -20.0 * eps && is == 1)
+                            on = std::sqrt((i - N1 / 2.0) * (i - N1 / 2.0) + (j - N2 / 2.0) * (j - N2 / 2.0) +
+                                           (k - N3 / 2.0) * (k - N3 / 2.0)) <= xprd / 20.0;
+                        else if (dk4 < eps && dk4 > -eps && is == 0)
+                            on = std::sqrt((i - N1 / 2.0) * (i - N1 / 2.0) + (j - N2 / 2.0) * (j - N2 / 2.0) +
+                                           (k - N3 / 4.0) * (k - N3 / 4.0)) <= xprd / 20.0;
+                        break;
+                    }
+                    case 6: {
+                        const double eps = 1e-2;
+                        on = dk2 < eps && dk2 > -eps && i > N1 / 4.0 && i < 3.0 * N1 / 4.0 && is == 0;
+                        break;
+                    }
+                    case 7: {
+                        const double eps = 1e-2;
+                        mask = i < 5 || j < 5 || k < 5;
+                        if (dk2 < eps && dk2 > -eps) {
+                            const double ir = (i - 5.0) * (i - 5.0) + (double)((j - N2 / 2) * (j - N2 / 2));
+                            if (ir <= 25 && (i == 5 || i == 6)) {
+                                mask = true;
+                                on = is == 0;
+                            }
+                        }
+                        break;
+                    }
+                    }
+                    xi[2 * idx] = on ? 1.0 : 0.0;
+                    xi[2 * idx + 1] = 0.0;
+                    if (xo) xo[idx] = mask ? 1.0 : 0.0;
+                }
+    return true;
 }
 
 void init_sxtal_straight_ortho(const Grid &g, double *xi)

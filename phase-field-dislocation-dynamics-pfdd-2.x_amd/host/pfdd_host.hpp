@@ -129,6 +129,12 @@ void init_sxtal_frank_read(const Grid &g, double *xi, double *xo);
 void init_sxtal_edge(const Grid &g, double *xi);
 // initial_sxtal_StraightNonOrtho (fft_fftw_slab.cpp:2804-2855).
 void init_sxtal_straight_nonortho(const Grid &g, double *xi);
+// mode1, mode2, 1LOrthoFCC, 1LNonOrthoFCC, 1LInclinedFCC, 2LOrtho, 3D3ShcpBasal,
+// 3D3ShcpNotch (fft_fftw_slab.cpp:2245-2742, 2864-3029); xo (may be NULL) gets
+// the notch's mask.  sa = scale a, xprd = box length along x.  False for
+// other names.
+bool init_sxtal_more(const std::string &style, const Grid &g, const Geometry &geo, double sa, double xprd, double *xi,
+                     double *xo);
 // initial_sxtal_StraightOrtho (fft_fftw_slab.cpp:2745-2800).
 void init_sxtal_straight_ortho(const Grid &g, double *xi);
 // initial_sxtal_1LOrthoBCC (fft_fftw_slab.cpp:2600-2660); sa, sb = scale command.

@@ -167,6 +167,7 @@ int pfdd_host_init_sxtal(const char *style, int nx, int ny, int nz, int ns, doub
     else if (s == "3SBCC") init_sxtal_3sbcc(g, xi);
     else if (s == "StraightNonOrtho") init_sxtal_straight_nonortho(g, xi);
     else if (s == "StraightOrtho") init_sxtal_straight_ortho(g, xi);
+    else if (init_sxtal_more(s, g, Geometry(), 1.0, (double)nx, xi, xo)) return 0;
     else if (s == "1LOrthoBCC") init_sxtal_1lorthobcc(g, 1.0, 1.0, xi);
     else return 1;
     if (xo && s != "frank_read")

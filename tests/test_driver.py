@@ -378,3 +378,31 @@ def test_every_reference_app_style_parses(tmp_path, app):
         assert np.allclose(xn2, wn, rtol=1e-12, atol=1e-15) and np.allclose(xb2, wb, rtol=1e-12, atol=1e-15)
     else:
         assert np.array_equal(xn2, xn) and np.array_equal(xb2, xb)
+
+
+@pytest.mark.parametrize("style,S,systems", [
+    ("mode1", 2, {0, 1}), ("mode2", 2, {0, 1}), ("1LOrthoFCC", 1, {0}), ("1LNonOrthoFCC", 1, {0}),
+    ("1LInclinedFCC", 1, {0}), ("2LOrtho", 2, {0, 1}), ("3D3ShcpBasal", 3, {0}), ("3D3ShcpNotch", 3, {0}),
+    ("StraightOrtho", 2, {0}), ("StraightNonOrtho", 1, {0}), ("1LOrthoBCC", 1, {0})])
+def test_reference_initial_configurations(style, S, systems):
+    """The hard-coded initial fields of FFTW_Slab (fft_fftw_slab.cpp:2196-3247):
+    zero imaginary part, Re xi in {0, 1}, set only on the systems the
+    reference seeds; single-plane seeds lie on one plane (k = nz/2 for the
+    {100}/{0001} loops, i + j + k = nx for the inclined {111} loop); the
+    notch also freezes the faces i, j, k < 5."""
+    import pfdd_setup
+    n = 96 if style == "mode2" else 32  # mode2 centres its loops for a 96^3 grid
+    with_mask = style == "3D3ShcpNotch"
+    out = pfdd_setup.sxtal(style, (n, n, n), S, with_mask=with_mask)
+    xi, xo = out if with_mask else (out, None)
+    assert np.all(xi.imag == 0) and set(np.unique(xi.real)) <= {0.0, 1.0}
+    assert {a for a in range(S) if xi[a].real.any()} == systems
+    i, j, k = np.nonzero(xi[0].real)
+    if style in ("1LOrthoFCC", "1LNonOrthoFCC", "3D3ShcpBasal", "StraightOrtho", "StraightNonOrtho", "1LOrthoBCC",
+                 "mode1", "3D3ShcpNotch"):
+        assert np.all(k == n // 2)
+    if style in ("1LInclinedFCC", "mode2"):
+        assert np.all(i + j + k == n)
+    if with_mask:
+        assert xo[:, :5].all() and xo[:, :, :5].all() and xo[:, :, :, :5].all()
+        assert np.all(xo[0][i, j, k] == 1)
