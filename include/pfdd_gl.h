@@ -209,6 +209,14 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out);
 int pfdd_gl_tape_backward(pfdd_gl *h, const double *dxi_T, double *dxi_0, double *dtau, double *dcoef);
 int pfdd_gl_tape_backward_device(pfdd_gl *h, const void *dxi_T, void *dxi_0, double *dtau, double *dcoef);
 
+/* gradient() (fft_fftw_slab.cpp:869-1065): the dislocation character angle
+ * theta of every system and cell (S*N doubles, [a][i][j][k]; -1 where the
+ * line direction |grad Re xi x n| <= 0.1), computed as the reference does at
+ * the start of the last GL iteration, i.e. from the field before its update
+ * (the current field before the first step); the in-place transform to
+ * global axes of :1020-1029 included.  What the apps' compute_theta dumps. */
+int pfdd_gl_theta(pfdd_gl *h, double *theta);
+
 /* Autotuner (no counterpart in the reference): times each candidate LDS tile
  * shape (0, 1, 2: about 1K/2K/4K FP64 or 2K/4K/8K FP32 elements per tile) of
  * the four line passes -- forward z, forward y, inverse y, inverse z + update

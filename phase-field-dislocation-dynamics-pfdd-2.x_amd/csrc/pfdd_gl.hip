@@ -835,7 +835,7 @@ struct ThetaParams {
     int nx, ny, nz, S;
 };
 
-template <typename R>
+template <typename R, bool THETA = false>
 __global__ void __launch_bounds__(NTHR)
     k_theta_cof(const typename cvec<R>::type *__restrict__ xo, R *__restrict__ cof, ThetaParams T, long long N,
                 const int *__restrict__ done)
@@ -858,17 +858,17 @@ __global__ void __launch_bounds__(NTHR)
     const double *n = T.xn + 3 * as, *b = T.xb + 3 * as;
     double t0 = gy * n[2] - gz * n[1], t1 = gz * n[0] - gx * n[2], t2 = gx * n[1] - gy * n[0];
     const double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-    double c = 1.0;
+    double c = 1.0, th = -1.0; // theta = -1: no line through the cell
     if (mag > 0.1) {
         t0 /= mag;
         t1 /= mag;
         t2 /= mag;
-        double th = acos(t0 * b[0] + t1 * b[1] + t2 * b[2]);
+        th = acos(t0 * b[0] + t1 * b[1] + t2 * b[2]);
         const double dir = (b[1] * t2 - b[2] * t1) * n[0] + (b[2] * t0 - b[0] * t2) * n[1] + (b[0] * t1 - b[1] * t0) * n[2];
         if (dir < 0.0) th = acos(-1.0) - th;
         c = th < 1.23 ? 1.1603 * th * th - 2.0431 * th + 1 : 0.5473 * th * th - 2.0035 * th + 1.8923;
     }
-    cof[(long long)a * N + r] = (R)c;
+    cof[(long long)a * N + r] = (R)(THETA ? th : c);
 }
 
 // ------------------------------------------ adjoint of pass 5 (reverse mode)
@@ -2335,18 +2335,19 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
                 return bail(fail(PFDD_ERR_HIP, "amp upload failed"));
         }
     }
-    if (d->core_model == PFDD_CORE_BCC) {
-        if ((rc = alloc(&h->cof, (size_t)h->SM * h->N * (h->prec == PFDD_FP64 ? 8 : 4)))) return bail(rc);
-        for (int q = 0; q < 9; ++q) h->TP.four[q] = d->four[q];
-        for (int q = 0; q < 3 * d->ns; ++q) {
-            h->TP.xn[q] = d->xn[q];
-            h->TP.xb[q] = d->xb[q];
-        }
-        h->TP.nx = h->nx;
-        h->TP.ny = h->ny;
-        h->TP.nz = h->nz;
-        h->TP.S = h->S;
+    // gradient()'s geometry: the character angle of core flag 5 and of
+    // pfdd_gl_theta
+    for (int q = 0; q < 9; ++q) h->TP.four[q] = d->four[q];
+    for (int q = 0; q < 3 * d->ns; ++q) {
+        h->TP.xn[q] = d->xn[q];
+        h->TP.xb[q] = d->xb[q];
     }
+    h->TP.nx = h->nx;
+    h->TP.ny = h->ny;
+    h->TP.nz = h->nz;
+    h->TP.S = h->S;
+    if (d->core_model == PFDD_CORE_BCC)
+        if ((rc = alloc(&h->cof, (size_t)h->SM * h->N * (h->prec == PFDD_FP64 ? 8 : 4)))) return bail(rc);
     rc = h->prec == PFDD_FP64 ? build_B<double>(h) : build_B<float>(h);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(h->stream);
@@ -2356,6 +2357,40 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
         if (rc) return bail(rc);
     }
     *out = h;
+    return PFDD_OK;
+}
+
+int pfdd_gl_theta(pfdd_gl *h, double *theta)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!theta) return fail(PFDD_ERR_ARG, "null theta");
+    // gradient() runs at the start of an iteration on the field before its
+    // update: the previous ping-pong buffer once the handle has stepped
+    const void *pre = h->stepped ? h->xi[1 - h->cur] : h->xi[h->cur];
+    const size_t n = (size_t)h->SM * h->N;
+    void *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, n * sizeof(double)));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+    if (h->prec == PFDD_FP64)
+        hipLaunchKernelGGL((k_theta_cof<double, true>), dim3((unsigned)((h->N + NTHR - 1) / NTHR), h->SM), dim3(NTHR),
+                           0, h->stream, (const double2 *)pre, (double *)buf, h->TP, h->N, h->done);
+    else
+        hipLaunchKernelGGL((k_theta_cof<float, true>), dim3((unsigned)((h->N + NTHR - 1) / NTHR), h->SM), dim3(NTHR),
+                           0, h->stream, (const float2 *)pre, (float *)buf, h->TP, h->N, h->done);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) {
+        if (h->prec == PFDD_FP64) {
+            e = hipMemcpy(theta, buf, n * sizeof(double), hipMemcpyDeviceToHost);
+        } else {
+            std::vector<float> f(n);
+            e = hipMemcpy(f.data(), buf, n * sizeof(float), hipMemcpyDeviceToHost);
+            for (size_t q = 0; q < n; ++q) theta[q] = f[q];
+        }
+    }
+    hipFree(buf);
+    if (e != hipSuccess) return fail(PFDD_ERR_HIP, "theta: %s", hipGetErrorString(e));
     return PFDD_OK;
 }
 

@@ -75,6 +75,12 @@ void FFT_HIP::project_core_energy()
     }
 }
 
+void FFT_HIP::theta(std::vector<double> &th)
+{
+    th.assign((size_t)slip_systems * nx * ny * nz, 0.0);
+    if (stepped) check(pfdd_gl_theta(h_, th.data()));
+}
+
 void FFT_HIP::cond_init()
 {
     // FFTW_Slab::allocate's condxi (zero) and backwardcond's layer algebra,
@@ -213,6 +219,7 @@ void FFT_HIP::resolSS()
 int FFT_HIP::iterate_block(int n, double tol)
 {
     check(pfdd_gl_step(h_, n, tol, &last));
+    stepped = true;
     xinorm = last.xinorm;
     xiave = last.xiave;
     E_core = last.e_core_ref;
@@ -329,6 +336,17 @@ void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
     const long long N = (long long)fft.nx * fft.ny * fft.nz;
     const int size_one = (int)d.fields.size();
     std::vector<double> buf((size_t)N * size_one);
+    // compute_theta / compute_usfe of the apps that have them (system 0 for
+    // theta, system n-1 of material->A_mpea for usfeN); app.h:130-131: 0 otherwise
+    static const char *theta_apps[] = {"1L1S111Bbcc", "1L1S100B", "1L1S110Binclined", "1SBCC", "3SBCC",
+                                       "BCCEdge", "BCCEdgePlanar", "cond100B", "FrankRead", "frank_read"};
+    static const char *usfe_apps[] = {"1SBCC", "3SBCC", "BCCEdge", "BCCEdgePlanar"};
+    bool has_theta = false, has_usfe = false;
+    for (auto a : theta_apps) has_theta |= app.style == a;
+    for (auto a : usfe_apps) has_usfe |= app.style == a;
+    std::vector<double> th;
+    for (auto &f : d.fields)
+        if (f == "theta" && has_theta && th.empty()) fft.theta(th);
     for (long long r = 0; r < N; ++r) {
         const int i = (int)(r / ((long long)fft.ny * fft.nz)), j = (int)((r / fft.nz) % fft.ny), k = (int)(r % fft.nz);
         for (int c = 0; c < size_one; ++c) {
@@ -340,6 +358,11 @@ void Output::write_dump(DumpSpec &d, App &app, FFT_HIP &fft, double time)
             else if (f.rfind("xir", 0) == 0 || f.rfind("xii", 0) == 0) {
                 const int a = std::atoi(f.c_str() + 3) - 1;
                 v = fft.xi[2 * (r + (long long)a * N) + (f[2] == 'r' ? 0 : 1)];
+            } else if (f == "theta") {
+                v = th.empty() ? 0.0 : th[r];
+            } else if (f.rfind("usfe", 0) == 0) {
+                const long long q = r + (long long)(f[4] - '1') * N;
+                v = (has_usfe && q < (long long)fft.mat.A_mpea.size()) ? fft.mat.A_mpea[q] : 0.0;
             } else if (f == "deltao" || f == "ddelta") {
                 // App1L1S100B / Appcond100B::compute_delta, compute_ddelta
                 // (app_1L1S_100B.cpp:348-389); other apps return 0 (app.h:132-133)
@@ -826,9 +849,11 @@ void Input::command(const std::vector<std::string> &w)
             bool t = false;
             for (auto q : tens) t |= f == q;
             const bool ok = t || f == "x" || f == "y" || f == "z" || f.rfind("xir", 0) == 0 ||
-                            f.rfind("xii", 0) == 0 || f == "deltao" || f == "ddelta";
+                            f.rfind("xii", 0) == 0 || f == "deltao" || f == "ddelta" || f == "theta" ||
+                            f == "usfe1" || f == "usfe2" || f == "usfe3";
             if (!ok)
-                deck_error("dump field " + f + " is not available (x y z xirN xiiN pxx..pyz exx..eyz deltao ddelta)");
+                deck_error("dump field " + f +
+                           " is not available (x y z theta usfe1-3 deltao ddelta pxx..pyz exx..eyz xirN xiiN)");
             if (t) fft.want_fields = true;
             d.fields.push_back(f);
         }

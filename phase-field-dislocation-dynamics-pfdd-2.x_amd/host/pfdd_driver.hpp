@@ -85,6 +85,10 @@ class FFT_HIP {
     // 1550-1624), the arrays the dump fields deltao / ddelta read
     std::vector<double> delta, ddelta;
     void project_core_energy();
+    // fft->theta (gradient(), computed at the start of each iteration); zero
+    // before the first iteration, as the freshly allocated array reads
+    bool stepped = false;
+    void theta(std::vector<double> &th);
     Material matB;
     bool have_matB = false, cond = false;
     double temperature = 300.0;

@@ -95,7 +95,7 @@ EXPORTS = [
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
     "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
-    "pfdd_gl_autotune", "pfdd_gl_set_tuning",
+    "pfdd_gl_autotune", "pfdd_gl_set_tuning", "pfdd_gl_theta",
     "pfdd_gl_cond_create", "pfdd_gl_cond_destroy", "pfdd_gl_cond_set", "pfdd_gl_cond_get", "pfdd_gl_cond_step",
 ]
 
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_graph_destroy": (c_int, [vp]),
         "pfdd_gl_autotune": (c_int, [vp, c_int, ip]),
         "pfdd_gl_set_tuning": (c_int, [vp, ip]),
+        "pfdd_gl_theta": (c_int, [vp, _dp]),
         "pfdd_gl_cond_create": (c_int, [ctypes.POINTER(CondDesc), ctypes.POINTER(vp)]),
         "pfdd_gl_cond_destroy": (c_int, [vp]),
         "pfdd_gl_cond_set": (c_int, [vp, _dp]),
@@ -326,6 +327,12 @@ class GLSolver:
         _check(self._lib.pfdd_gl_profile(self._h, int(nsteps), _ptr(ms), ctypes.byref(nk)))
         names = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update", "reduce", "step"]
         return dict(zip(names, ms.tolist()))
+
+    def theta(self):
+        """Character angle of gradient() for every system and cell (pfdd_gl_theta)."""
+        out = np.zeros((self.SM,) + tuple(self.shape))
+        _check(self._lib.pfdd_gl_theta(self._h, _ptr(out)))
+        return out
 
     def traffic(self):
         k = np.zeros(5)

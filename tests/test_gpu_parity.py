@@ -275,3 +275,32 @@ def test_gpu_matches_committed_fixtures():
         sc = fx[name + "/scalars"]
         for v, w in zip((r["xinorm"], r["xiave"], r["e_total"]), sc[1:]):
             assert scal_rel(v, w) < 1e-9, (name, v, w)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_theta_matches_oracle(precision):
+    """pfdd_gl_theta (gradient(), fft_fftw_slab.cpp:869-1065) against the
+    oracle: of the current field before any step, then of the field before
+    the last update.  acos amplifies rounding near |cos| = 1 and the
+    |tang| > 0.1 switch can flip on a rounding difference, so a few cells may
+    differ; all others agree within 1e-9 (FP64) / 1e-3 (FP32)."""
+    _say(f"case: theta of a skewed 32x32x16 x 3 field, precision {precision}")
+    prim = ((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1))
+    case = P.make_case(shape=(32, 32, 16), noise=2e-2, prim_vectors=prim,
+                       loops=[(0, 2, 8, 16, 16, 6), (1, 2, 8, 10, 20, 4)])
+    g = P.gpu_solver(case, precision)
+    tol = 1e-9 if precision == 0 else 1e-3
+    try:
+        xi_prev = case["xi0"]
+        for nsteps in (0, 3):
+            if nsteps:
+                g.step(nsteps - 1)
+                xi_prev = g.get_xi()
+                g.step(1)
+            want = oracle.theta(xi_prev, case["geo"]["four"], case["xn"], case["xb"]).reshape(g.theta().shape)
+            got = g.theta()
+            bad = np.abs(got - want) > tol
+            assert bad.mean() < 2e-3, (nsteps, bad.mean())
+            assert np.any(want != -1.0) and np.any(want == -1.0)
+    finally:
+        g.close()
