@@ -222,8 +222,9 @@ int pfdd_gl_theta(pfdd_gl *h, double *theta);
  * the four line passes -- forward z, forward y, inverse y, inverse z + update
  * -- and both x-pass variants (0: one tile per workgroup, 1: persistent
  * workgroups that load the next tile while transforming the current one)
- * over `nsteps` GL iterations on this handle's own buffers, and keeps the
- * fastest per kernel (choice[5], may be NULL).  The field and step counters
+ * on this handle's own buffers: after 20 warm-up iterations, two interleaved
+ * rounds of `nsteps` GL iterations per candidate, keeping each kernel's best
+ * time; the fastest per kernel wins (choice[5], may be NULL).  The field and step counters
  * are restored; the choice is cached for the process per grid, system count,
  * batch, precision and options.  Results (xi) do not depend on the choice.
  * The update of core models 2, 4 and 9 always uses tile shape 0. */

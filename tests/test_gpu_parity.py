@@ -2,7 +2,7 @@
 
 Tolerances (floating point, written per test):
   FP64 plain path: xi, energies and xinorm within 1e-9 relative of the oracle
-                   after the stated number of iterations (north-star bar);
+                   after the stated number of iterations;
   FP32 mode:       within 1e-5 relative.
 Relative error of a field = max|gpu - ref| / max|ref|.
 """
