@@ -74,7 +74,8 @@ typedef struct pfdd_gl pfdd_gl;
 /* Everything FFTW_Slab::init/setup (fft_fftw_slab.cpp:52-119) and the app's
  * set_slip/constructor supply to the GL loop. */
 typedef struct pfdd_gl_desc {
-    int nx, ny, nz;      /* grid: fft->nx, ny, nz (create_sites.cpp:228-232)           */
+    int nx, ny, nz;      /* grid: fft->nx, ny, nz (create_sites.cpp:228-232); each one of
+                            8 16 32 40 48 64 80 96 128 160 192 256 512                  */
     int ns;              /* slip systems: app->slip_systems                             */
     int precision;       /* PFDD_FP64 (plain) or PFDD_FP32                              */
     double four[9];      /* reciprocal matrix fft->four, row-major (app.cpp:396-431)    */

@@ -162,18 +162,54 @@ template <bool INV, typename V> __device__ __forceinline__ void dft3(V *v)
     }
 }
 
+// Length-5 DFT from the symmetric pairs t = x1 + x4, x2 + x3 and the
+// antisymmetric ones x1 - x4, x2 - x3 (c_k = cos(2 pi k/5), s_k = sin(2 pi k/5)):
+//   X0 = x0 + t1 + t2,  a1 = x0 + c1 t1 + c2 t2,  a2 = x0 + c2 t1 + c1 t2,
+//   b1 = s1 d1 + s2 d2, b2 = s2 d1 - s1 d2,
+//   forward: X1 = a1 - i b1, X4 = a1 + i b1, X2 = a2 - i b2, X3 = a2 + i b2;
+//   inverse: the signs of i swap.
+template <bool INV, typename V> __device__ __forceinline__ void dft5(V *v)
+{
+    typedef decltype(V{}.x) R;
+    const R c1 = (R)0.309016994374947424102293417182819059;  // cos(2 pi/5)
+    const R c2 = (R)-0.809016994374947424102293417182819059; // cos(4 pi/5)
+    const R s1 = (R)0.951056516295153572116439333379382143;  // sin(2 pi/5)
+    const R s2 = (R)0.587785252292473129168705954639072769;  // sin(4 pi/5)
+    const V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const V d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    const V x0 = v[0];
+    const V a1 = mk<V>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+    const V a2 = mk<V>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+    const V b1 = mk<V>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+    const V b2 = mk<V>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+    const V ib1 = mk<V>(-b1.y, b1.x), ib2 = mk<V>(-b2.y, b2.x); // i b
+    v[0] = cadd(x0, cadd(t1, t2));
+    if (INV) {
+        v[1] = cadd(a1, ib1);
+        v[4] = csub(a1, ib1);
+        v[2] = cadd(a2, ib2);
+        v[3] = csub(a2, ib2);
+    } else {
+        v[1] = csub(a1, ib1);
+        v[4] = cadd(a1, ib1);
+        v[2] = csub(a2, ib2);
+        v[3] = cadd(a2, ib2);
+    }
+}
+
 template <int RAD, bool INV, typename V> __device__ __forceinline__ void dft(V *v)
 {
     if constexpr (RAD == 2) dft2<INV>(v);
     else if constexpr (RAD == 3) dft3<INV>(v);
     else if constexpr (RAD == 4) dft4<INV>(v);
+    else if constexpr (RAD == 5) dft5<INV>(v);
     else if constexpr (RAD == 8) dft8<INV>(v);
     else static_assert(RAD == 2, "unsupported radix");
 }
 
 constexpr int pick_radix(int m)
 {
-    return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m;
+    return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m % 5 == 0 ? 5 : m;
 }
 
 // Lines per LDS tile for a transform of length N: a power of two giving about
@@ -242,7 +278,7 @@ __device__ __forceinline__ void fft_stages(const Tile<V> lds, const V *__restric
 {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix(N / NS);
-        static_assert(RAD == 2 || RAD == 3 || RAD == 4 || RAD == 8, "length must be 2^a 3^b");
+        static_assert(RAD == 2 || RAD == 3 || RAD == 4 || RAD == 5 || RAD == 8, "length must be 2^a 3^b 5^c");
         stockham_stage<N, RAD, NS, NL, NTHR, INV, V>(lds, tw, tid);
         fft_stages<N, NS * RAD, NL, NTHR, INV, V>(lds, tw, tid);
     }

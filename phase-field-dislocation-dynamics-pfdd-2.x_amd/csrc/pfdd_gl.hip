@@ -281,6 +281,9 @@ template <typename R, int N, int S> constexpr int mid_pairs()
     constexpr int target = sizeof(R) == 8 ? (S == 12 ? 4096 : 2048) : 4096;
     int np = 1;
     while (np * 2 * S * N <= target) np *= 2;
+    // the padded tile stays within 60 KB of LDS (the constants and the
+    // reduction scratch of the pass come on top)
+    while (np > 1 && (long long)N * (S * np + 1) * 2 * (int)sizeof(R) > 61440) np /= 2;
     return np;
 }
 
@@ -1198,7 +1201,7 @@ __global__ void __launch_bounds__(NTHR)
 }
 
 // ------------------------------------------------------------------ dispatch
-#define PFDD_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
+#define PFDD_SIZES(X) X(8) X(16) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(512)
 
 bool size_supported(int n)
 {
@@ -1336,7 +1339,7 @@ int mid_blocks(int n, int S, long long ncol)
 
 template <typename R, int N, int S> constexpr bool mid_fits()
 {
-    return (long long)N * (S * mid_pairs<R, N, S>() + 1) * 2 * sizeof(R) <= 65536;
+    return (long long)N * (S * mid_pairs<R, N, S>() + 1) * 2 * sizeof(R) <= 61440;
 }
 
 template <typename R, int S>
@@ -2206,7 +2209,7 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     if (nbatch < 1) return fail(PFDD_ERR_ARG, "nbatch must be >= 1");
     if ((long long)nbatch * d->ns > 4096) return fail(PFDD_ERR_ARG, "nbatch * ns must be <= 4096");
     if (!size_supported(d->nx) || !size_supported(d->ny) || !size_supported(d->nz))
-        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each dimension must be one of 8,16,32,48,64,96,128,256,512",
+        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each dimension must be one of 8,16,32,40,48,64,80,96,128,160,192,256,512",
                     d->nx, d->ny, d->nz);
     if (!systems_supported(d->ns))
         return fail(PFDD_ERR_UNSUPPORTED, "slip_systems=%d: supported counts are 1,2,3,4,6,12", d->ns);
@@ -3049,7 +3052,7 @@ int pfdd_gl_cond_create(const pfdd_gl_cond_desc *d, pfdd_gl_cond **out)
     if (!d || !out) return fail(PFDD_ERR_ARG, "null argument");
     *out = nullptr;
     if (!size_supported(d->nx) || !size_supported(d->ny) || !size_supported(d->nz))
-        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each size must be one of 8 16 32 48 64 96 128 256 512",
+        return fail(PFDD_ERR_UNSUPPORTED, "grid %dx%dx%d: each size must be one of 8 16 32 40 48 64 80 96 128 160 192 256 512",
                     d->nx, d->ny, d->nz);
     if (!d->sigma || !d->dsigma) return fail(PFDD_ERR_ARG, "null layer conductivity arrays");
     if (!(d->El != 0.0) || !(d->epsilonzero > 0.0))

@@ -81,7 +81,8 @@ def test_stored_B_fp32_and_skewed():
     run_pair(case, 30, precision=1, check_every=10, tol_field=1e-5, tol_scal=1e-5, stored_B=True)
 
 
-@pytest.mark.parametrize("shape", [(48, 16, 32), (16, 96, 8), (8, 32, 48)])
+@pytest.mark.parametrize("shape", [(48, 16, 32), (16, 96, 8), (8, 32, 48), (40, 16, 80), (16, 160, 8),
+                                   (192, 8, 16), (80, 40, 8)])
 def test_mixed_radix_grids(shape):
     _say(f"case: {shape} x 3 basal, 10 steps, FP64")
     nx, ny, nz = shape
@@ -304,3 +305,10 @@ def test_theta_matches_oracle(precision):
             assert np.any(want != -1.0) and np.any(want == -1.0)
     finally:
         g.close()
+
+
+def test_radix5_fcc12_grid():
+    _say("case: 40x40x16 x 12 fcc (radix 5 with the two-column x pass), 8 steps, FP64 and FP32")
+    case = P.make_case(shape=(40, 40, 16), systems="fcc12", noise=1e-2, sigma_mu=0.004)
+    run_pair(case, 8, check_every=4)
+    run_pair(case, 8, precision=1, tol_field=1e-5, tol_scal=1e-5)
