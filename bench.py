@@ -304,7 +304,8 @@ def bench_ensemble(args):
 def bench_adjoint(args):
     """Reverse mode on the 64^3 x 3 loop problem: 500 GL iterations forward with
     checkpoints every 50, then the adjoint sweep; cost ratio and a central
-    finite-difference check of dL/dcoef and dL/dsigma (h = 1e-6)."""
+    finite-difference check of dL/dcoef and dL/dsigma (h = 1e-6; its O(h^2)
+    truncation also removed by Richardson extrapolation with h/2)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     n, nsteps, every = 64, args.steps, 50
     m = setup.cu_like()
@@ -366,27 +367,41 @@ def bench_adjoint(args):
                "backward_ms_per_step": 1e3 * t_bwd / nsteps, "adjoint_cost_vs_forward": t_bwd / t_fwd,
                "loss": loss(xT)}
         if prec == pfdd_gl.PFDD_FP64:
-            h, worst, fds, adj = 1e-6, 0.0, [], []
-            for a in range(S):
-                for q in range(4):
-                    cp, cm = c_guess.copy(), c_guess.copy()
-                    cp[a, q] += h
-                    cm[a, q] -= h
-                    fd = (loss(forward(cp, sig_mu)) - loss(forward(cm, sig_mu))) / (2 * h)
-                    worst = max(worst, abs(fd - dcoef[a, q]) / max(abs(fd), 1e-30))
+            h, worst, fds, adj, rich = 1e-6, 0.0, [], [], []
+
+            def cfd(dc, ds, hh):
+                return (loss(forward(c_guess + hh * dc, sig_mu + hh * ds)) -
+                        loss(forward(c_guess - hh * dc, sig_mu - hh * ds))) / (2 * hh)
+
+            for a in range(S + 1):
+                for q in range(4 if a < S else 1):
+                    dc, ds = np.zeros_like(c_guess), 0.0
+                    if a < S:
+                        dc[a, q] = 1.0
+                    else:
+                        ds = 1.0
+                    fd, fd2 = cfd(dc, ds, h), cfd(dc, ds, h / 2)
+                    g = dcoef[a, q] if a < S else dsig
+                    worst = max(worst, abs(fd - g) / max(abs(fd), 1e-30))
                     fds.append(fd)
-                    adj.append(dcoef[a, q])
-            fd = (loss(forward(c_guess, sig_mu + h)) - loss(forward(c_guess, sig_mu - h))) / (2 * h)
-            worst = max(worst, abs(fd - dsig) / abs(fd))
-            fds.append(fd)
-            adj.append(dsig)
+                    adj.append(g)
+                    # Richardson: the O(h^2) truncation of the central difference cancels
+                    rich.append((4.0 * fd2 - fd) / 3.0)
 
             def normwise(x, y):
                 x, y = np.asarray(x), np.asarray(y)
                 return float(np.linalg.norm(x - y) / np.linalg.norm(x))
+            fa, aa = np.asarray(fds), np.asarray(adj)
             res["fd_check"] = {"h": h, "params": "12 GSF coefficients + applied stress",
                                "max_rel_diff_adjoint_vs_central_fd": worst,
-                               "normwise_rel_diff": normwise(fds, adj)}
+                               "normwise_rel_diff": normwise(fds, adj),
+                               # per component, scaled by the largest gradient component
+                               "max_abs_diff_over_max_grad": float(np.max(np.abs(fa - aa)) / np.max(np.abs(fa))),
+                               # rounding floor of a central difference of the loss: eps L / h
+                               "fd_rounding_floor_abs": float(np.finfo(float).eps * res["loss"] / h),
+                               "min_abs_grad": float(np.min(np.abs(fa))),
+                               "max_rel_diff_adjoint_vs_richardson_fd": float(
+                                   np.max(np.abs(np.asarray(rich) - aa) / np.abs(np.asarray(rich))))}
             out["grad64"] = (dcoef.copy(), dsig)
         else:
             g64, s64 = out["grad64"]
