@@ -234,7 +234,8 @@ def bench_ensemble(args):
     for g in groups:
         s = pfdd_gl.GLSolver(g["shape"], g["xn"], g["xb"], g["four"], g["C44"], g["C12"], g["C11"], g["dslip"],
                              g["cddt"], S12=g["S12"], S44=g["S44"], core_model=pfdd_gl.CORE_SERIES,
-                             coef=g["coef"], ncoef=g["ncoef"], amp=g["amp"], precision=prec, batch=g["batch"])
+                             coef=g["coef"], ncoef=g["ncoef"], amp=g["amp"], precision=prec, batch=g["batch"],
+                             autotune=True)
         s.set_xi(g["xi0"])
         s.set_tau(g["tau"])
         solvers.append(s)
