@@ -151,6 +151,7 @@ struct GParams {
     double four[9];
     double inv2C44, inv1mnu, invmu;
     double C44x2, C12; // isotropic stiffness: C:Q = 2 C44 Q + C12 tr(Q) I
+    double inv_n[3];   // 1/nx, 1/ny, 1/nz (frequencies by multiplication in the x pass)
     int nx, ny, nz;
 };
 
@@ -452,8 +453,13 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
 template <int N, int NP, int S, typename T>
 __device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p, T f[3], T &f2)
 {
-    const int k = (int)(col / G.ny), j = (int)(col % G.ny);
-    const double kx = frac_freq(p, G.nx), ky = frac_freq(j, G.ny), kz = frac_freq(k, G.nz);
+    const int c = (int)col; // column (k, j) of one member: ny * nz < 2^31
+    const int k = c / G.ny, j = c - k * G.ny;
+    // frec's k/n with the reciprocal of n (one rounding more than the
+    // division for sizes that are not powers of two): the x pass evaluates it
+    // for every k-point of every step
+    auto fr = [](int kk, int n, double inv) { return (double)(kk >= n / 2 ? kk - n : kk) * inv; };
+    const double kx = fr(p, G.nx, G.inv_n[0]), ky = fr(j, G.ny, G.inv_n[1]), kz = fr(k, G.nz, G.inv_n[2]);
     double fd[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) fd[c] = kx * G.four[c] + ky * G.four[3 + c] + kz * G.four[6 + c];
@@ -1952,6 +1958,9 @@ int build_B(pfdd_gl *h)
     G.nx = h->nx;
     G.ny = h->ny;
     G.nz = h->nz;
+    G.inv_n[0] = 1.0 / h->nx;
+    G.inv_n[1] = 1.0 / h->ny;
+    G.inv_n[2] = 1.0 / h->nz;
     if (!h->stored_B) return PFDD_OK;
     const long long np = (long long)S * (S + 1) / 2;
     HIPCHK(hipMalloc(&h->B, (size_t)np * h->N * sizeof(R)));
