@@ -950,15 +950,35 @@ __global__ void __launch_bounds__(NTHR)
         else sincospif((float)x.x, (float *)&sp, (float *)&cp);
         const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
         R sm = s1, cm = c1, d2 = (R)0;
+        if (P.model == PFDD_CORE_FOURIER || P.model == PFDD_CORE_PYRII) {
+            // Fourier density a0 + sum_m a_m cos(w x) + b_m sin(w x), w = 2 pi m:
+            //   dE = sum_m w (b_m cos - a_m sin),  dE' = -sum_m w^2 (a_m cos + b_m sin)
+            //   d(dE)/d a_m = -w sin(w x),  d(dE)/d b_m = w cos(w x)   (slots m, 4 + m)
+            // pyrII as executed: only the last cell of each system carries dE
+            if (P.model == PFDD_CORE_FOURIER || (idx % P.ncell) == P.ncell - 1) {
 #pragma unroll
-        for (int m = 1; m <= MAXM; ++m) {
-            if (m <= nm) {
-                const R c = (R)P.coef[a * MAXM + m - 1];
-                d2 += c * ((R)(2 * m * m) * (R)(M_PI * M_PI)) * cm;
-                acc[m] += (double)mr * (double)A * (double)m * M_PI * (double)sm;
-                const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
-                sm = sn;
-                cm = cn;
+                for (int m = 1; m <= 4; ++m) {
+                    const R am = (R)P.coef[a * MAXM + m - 1], bm = (R)P.coef[a * MAXM + m + 3];
+                    const double w = 2.0 * M_PI * m;
+                    d2 -= (R)(w * w) * (am * cm + bm * sm);
+                    acc[m] += (double)mr * (double)A * (-w * (double)sm);
+                    acc[4 + m] += (double)mr * (double)A * (w * (double)cm);
+                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
+                    sm = sn;
+                    cm = cn;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 1; m <= MAXM; ++m) {
+                if (m <= nm) {
+                    const R c = (R)P.coef[a * MAXM + m - 1];
+                    d2 += c * ((R)(2 * m * m) * (R)(M_PI * M_PI)) * cm;
+                    acc[m] += (double)mr * (double)A * (double)m * M_PI * (double)sm;
+                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
+                    sm = sn;
+                    cm = cn;
+                }
             }
         }
         V y;
@@ -2544,9 +2564,9 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
     if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
-    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_PYRII || h->P.model == PFDD_CORE_FOURIER ||
-        h->P.model == PFDD_CORE_BCC)
-        return fail(PFDD_ERR_UNSUPPORTED, "reverse mode supports the sine-series core models (0, 1, 3, 7, 8)");
+    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_BCC)
+        return fail(PFDD_ERR_UNSUPPORTED,
+                    "reverse mode supports the sine-series (0, 1, 3, 7, 8) and Fourier (4, 9) core models");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3]) : tile_lines_rt<float>(h->nz, h->tq[3]);
     if (((long long)h->nx * h->ny) % nl != 0)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);

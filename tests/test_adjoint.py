@@ -244,3 +244,66 @@ def test_torch_autograd_function():
     assert np.allclose(xi0.grad.numpy(), lam, rtol=1e-12, atol=1e-12 * np.max(np.abs(lam)))
     assert np.allclose(coef.grad.numpy(), dc, rtol=1e-12, atol=1e-14)
     assert abs(g - float(np.dot(dtau, r.numpy()))) <= 1e-10 * abs(g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("core", ["fourier", "pyrII"])
+def test_gpu_adjoint_fourier_models_match_gpu_fd(core):
+    """Reverse mode of the Fourier core models (flag 9 on every cell, flag 4
+    as executed: force on each system's last cell only) against central
+    differences of the GPU forward run: the shared coefficients aa_m, bb_m
+    (gradient = sum over systems), the stresses and a field direction."""
+    print(f"case: adjoint {core} core 16^3 x 3 vs central FD, 16 steps, every 4", flush=True)
+    import pfdd_gl
+    shape, nsteps, every = (16, 16, 16), 16, 4
+    case = P.make_case(shape=shape, noise=1e-2, sigma_mu=0.006, core=core,
+                       loops=[(0, 2, 8, 8.0, 8.0, 4)])
+    target = target_of(case)
+    m = case["mat"]
+    par = case["core_par"]
+
+    def solver(c):
+        gc = dict(case["gpu_core"], coef=c)
+        s = pfdd_gl.GLSolver(shape, case["xn"], case["xb"], case["geo"]["four"], m["C44"], m["C12"], m["C11"],
+                             m["dslip"], case["cddt"], S12=m["S12"], S44=m["S44"], **gc)
+        s.set_xi(case["xi0"])
+        s.set_tau(case["tau"])
+        return s
+
+    def run(c=par, xi0=None, tau=None):
+        s = solver(c)
+        if xi0 is not None:
+            s.set_xi(xi0)
+        if tau is not None:
+            s.set_tau(tau)
+        s.step(nsteps)
+        out = loss(s.get_xi(), target)
+        s.close()
+        return out
+
+    s = solver(par)
+    s.tape_record(nsteps, every)
+    xT = s.get_xi()
+    lam, dtau, dc = s.tape_backward(2.0 * (xT - target), 8)
+    s.close()
+    assert dc.shape == (3, 8)
+    h, worst = 1e-7, 0.0
+    for q in range(8):  # aa1..aa4, bb1..bb4 = par[1..8]
+        c1, c2 = par.copy(), par.copy()
+        c1[1 + q] += h
+        c2[1 + q] -= h
+        g = (run(c=c1) - run(c=c2)) / (2 * h)
+        worst = max(worst, abs(g - dc[:, q].sum()) / max(1.0, abs(g)))
+    for a in range(3):
+        t1, t2 = np.array(case["tau"], dtype=float), np.array(case["tau"], dtype=float)
+        t1[a] += h
+        t2[a] -= h
+        g = (run(tau=t1) - run(tau=t2)) / (2 * h)
+        worst = max(worst, abs(g - dtau[a]) / max(1.0, abs(g)))
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(case["xi0"].shape) + 1j * rng.standard_normal(case["xi0"].shape)
+    g = (run(xi0=case["xi0"] + h * v) - run(xi0=case["xi0"] - h * v)) / (2 * h)
+    pred = float(np.sum(lam.real * v.real + lam.imag * v.imag))
+    worst = max(worst, abs(g - pred) / max(1.0, abs(g)))
+    print(f"adjoint vs central FD ({core}): worst relative difference {worst:.2e}", flush=True)
+    assert worst <= 1e-6
