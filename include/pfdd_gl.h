@@ -203,9 +203,10 @@ int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9],
  *                       sine-series form dE = A sum_m c_m m pi sin(2 m pi x)
  *                       (PERFECT: m = 1 -> An'; SINE: m = 1, 2 -> isf', Cn';
  *                       FOURIER / PYRII: aa1..aa4, bb1..bb4, ncoef 8 -- aa0
- *                       does not enter the force;
+ *                       does not enter the force; EXTENDED: c0..c4, a1, a3,
+ *                       ncoef 7 -- c0 does not enter the force;
  *                       per system -- sum over a for a shared coefficient)
- * Core models 2 (extended) and 5 (bcc_perfect) have no reverse mode.
+ * Core model 5 (bcc_perfect) has no reverse mode.
  * Requires nx*ny to be a multiple of the inverse-z tile (every power-of-two
  * and 48/96 grid).  The _device variant takes device pointers (double2
  * layout, any precision handle). */

@@ -132,6 +132,46 @@ __device__ __forceinline__ void core_extended(R y1, R y2, R y3, const R *c, R &f
           a1 * (cs[0] - cs[2]) + (R)2 * a3 * (c2[0] - c2[2]));
 }
 
+// Reverse mode of core_extended.  The plane density is a sum of 21 terms
+// c_g T(2 pi u.x) over the plane's triple x = (x_0, x_1, x_2), T = cos for
+// the groups c1..c4 and sin for a1, a3, with the integer vectors u below (the
+// arguments A, B, D of core_extended, and 2A for c3 and a3).  For the
+// system k of the triple and mu = the triple's adjoint (masked) values:
+//   hv       = sum_i mu_i d2f/dx_k dx_i = -(2 pi)^2 sum_t c_g u_tk (u_t.mu) T_t
+//   gc[g]    = mu_k d(df/dx_k)/dc_g     = mu_k sum_{t in g} 2 pi u_tk T'_t
+// with T'_t = -sin for cosine terms and cos for sine terms (gc[0], the
+// constant c0, stays 0).
+template <typename R>
+__device__ __forceinline__ void core_extended_adj(const R x[3], const R mu[3], int k, const R *c, R &hv,
+                                                  double gc[7])
+{
+    // u vectors, coefficient index (1..6 = c1..c4, a1, a3) and type (0 cos, 1 sin)
+    constexpr signed char U[21][3] = {{1, -1, 0},  {0, 1, -1},  {-1, 0, 1},  {2, -1, -1}, {-1, 2, -1}, {-1, -1, 2},
+                                      {2, -2, 0},  {0, 2, -2},  {-2, 0, 2},  {3, -1, -2}, {3, -2, -1}, {-2, 3, -1},
+                                      {-1, 3, -2}, {-1, -2, 3}, {-2, -1, 3}, {1, -1, 0},  {0, 1, -1},  {-1, 0, 1},
+                                      {2, -2, 0},  {0, 2, -2},  {-2, 0, 2}};
+    constexpr signed char G[21] = {1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 4, 4, 4, 5, 5, 5, 6, 6, 6};
+    constexpr double tp = 2.0 * M_PI;
+    double h = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) gc[q] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 21; ++t) {
+        const R arg = (R)U[t][0] * x[0] + (R)U[t][1] * x[1] + (R)U[t][2] * x[2];
+        const R um = (R)U[t][0] * mu[0] + (R)U[t][1] * mu[1] + (R)U[t][2] * mu[2];
+        R sn, cs;
+        if constexpr (sizeof(R) == 8) sincospi(2.0 * (double)arg, (double *)&sn, (double *)&cs);
+        else sincospif(2.0f * (float)arg, (float *)&sn, (float *)&cs);
+        const bool is_sin = G[t] >= 5;
+        const double Tv = is_sin ? (double)sn : (double)cs;
+        const double Td = is_sin ? (double)cs : -(double)sn;
+        const double uk = (double)U[t][k];
+        h -= tp * tp * (double)c[G[t]] * uk * (double)um * Tv;
+        gc[G[t]] += (double)mu[k] * tp * uk * Td;
+    }
+    hv = (R)h;
+}
+
 struct BParams {
     double sig[MAXS * 9]; // sigma_a = C : eps_a
     double eps[MAXS * 9]; // eps_a = b_a n_a^T / dslip
@@ -891,6 +931,9 @@ __global__ void __launch_bounds__(NTHR)
 //   dL/dtau_a   += cddt sum_r Re mu_a
 //   dL/dc_{a,m} += -cddt sum_r Re mu_a A m pi sin(2 m pi Re xi_a)
 // with dE = A sum_m c_m m pi sin(2 m pi x), dE' = A sum_m c_m 2 m^2 pi^2 cos(2 m pi x).
+// The extended model couples the three systems of a plane: there dE'(Re xi)
+// Re mu becomes the Hessian of the plane density applied to the triple's
+// Re mu (core_extended_adj).
 // T1 holds mu after passes 1-4; this kernel applies the last inverse pass and
 // the update above, and leaves per-block sums of Re mu (slot 0) and of
 // Re mu A m pi sin(2 m pi x) (slot m) in gpart.  Every block covers lines of
@@ -949,8 +992,25 @@ __global__ void __launch_bounds__(NTHR)
         if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
         else sincospif((float)x.x, (float *)&sp, (float *)&cp);
         const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
-        R sm = s1, cm = c1, d2 = (R)0;
-        if (P.model == PFDD_CORE_FOURIER || P.model == PFDD_CORE_PYRII) {
+        R sm = s1, cm = c1, d2 = (R)0, hv = (R)0;
+        if (P.model == PFDD_CORE_EXTENDED) {
+            // the triple of systems of this plane at the same cell (no amp[])
+            const int j = a % 3;
+            const long long b0 = idx - (long long)j * P.ncell;
+            R y[3], m3[3], cc[7];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const long long iq = b0 + q * P.ncell;
+                y[q] = q == j ? x.x : xs[iq].x;
+                m3[q] = q == j ? mr : ((mask && mask[iq]) ? (R)0 : lam[iq].x);
+            }
+#pragma unroll
+            for (int q = 0; q < 7; ++q) cc[q] = (R)P.coef[a * MAXM + q];
+            double gc[7];
+            core_extended_adj<R>(y, m3, j, cc, hv, gc);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) acc[1 + q] += gc[q];
+        } else if (P.model == PFDD_CORE_FOURIER || P.model == PFDD_CORE_PYRII) {
             // Fourier density a0 + sum_m a_m cos(w x) + b_m sin(w x), w = 2 pi m:
             //   dE = sum_m w (b_m cos - a_m sin),  dE' = -sum_m w^2 (a_m cos + b_m sin)
             //   d(dE)/d a_m = -w sin(w x),  d(dE)/d b_m = w cos(w x)   (slots m, 4 + m)
@@ -982,7 +1042,7 @@ __global__ void __launch_bounds__(NTHR)
             }
         }
         V y;
-        y.x = lm.x - cddt * (g.x * invN + A * d2 * mr);
+        y.x = lm.x - cddt * (g.x * invN + A * d2 * mr + hv);
         y.y = lm.y - cddt * (g.y * invN);
         lam_out[idx] = y;
         acc[0] += (double)mr;
@@ -2564,9 +2624,9 @@ int pfdd_gl_tape_record(pfdd_gl *h, int nsteps, int every, pfdd_gl_reduce *out)
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1 || every < 1) return fail(PFDD_ERR_ARG, "need nsteps >= 1 and every >= 1");
     if (h->M > 1) return fail(PFDD_ERR_UNSUPPORTED, "reverse mode of a batch handle");
-    if (h->P.model == PFDD_CORE_EXTENDED || h->P.model == PFDD_CORE_BCC)
+    if (h->P.model == PFDD_CORE_BCC)
         return fail(PFDD_ERR_UNSUPPORTED,
-                    "reverse mode supports the sine-series (0, 1, 3, 7, 8) and Fourier (4, 9) core models");
+                    "reverse mode supports the sine-series (0, 1, 3, 7, 8), extended (2) and Fourier (4, 9) core models");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3]) : tile_lines_rt<float>(h->nz, h->tq[3]);
     if (((long long)h->nx * h->ny) % nl != 0)
         return fail(PFDD_ERR_UNSUPPORTED, "reverse mode needs nx*ny divisible by %d", nl);

@@ -247,12 +247,14 @@ def test_torch_autograd_function():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("core", ["fourier", "pyrII"])
+@pytest.mark.parametrize("core", ["fourier", "pyrII", "extended"])
 def test_gpu_adjoint_fourier_models_match_gpu_fd(core):
     """Reverse mode of the Fourier core models (flag 9 on every cell, flag 4
-    as executed: force on each system's last cell only) against central
-    differences of the GPU forward run: the shared coefficients aa_m, bb_m
-    (gradient = sum over systems), the stresses and a field direction."""
+    as executed: force on each system's last cell only) and of the extended
+    model (flag 2: the three systems of a plane coupled through one density)
+    against central differences of the GPU forward run: the shared
+    coefficients (aa_m, bb_m; c0..c4, a1, a3: gradient = sum over systems),
+    the stresses and a field direction."""
     print(f"case: adjoint {core} core 16^3 x 3 vs central FD, 16 steps, every 4", flush=True)
     import pfdd_gl
     shape, nsteps, every = (16, 16, 16), 16, 4
@@ -284,14 +286,15 @@ def test_gpu_adjoint_fourier_models_match_gpu_fd(core):
     s = solver(par)
     s.tape_record(nsteps, every)
     xT = s.get_xi()
-    lam, dtau, dc = s.tape_backward(2.0 * (xT - target), 8)
+    nc, off = (7, 0) if core == "extended" else (8, 1)  # c0..a3 = par[0..6]; aa1..bb4 = par[1..8]
+    lam, dtau, dc = s.tape_backward(2.0 * (xT - target), nc)
     s.close()
-    assert dc.shape == (3, 8)
+    assert dc.shape == (3, nc)
     h, worst = 1e-7, 0.0
-    for q in range(8):  # aa1..aa4, bb1..bb4 = par[1..8]
+    for q in range(nc):
         c1, c2 = par.copy(), par.copy()
-        c1[1 + q] += h
-        c2[1 + q] -= h
+        c1[off + q] += h
+        c2[off + q] -= h
         g = (run(c=c1) - run(c=c2)) / (2 * h)
         worst = max(worst, abs(g - dc[:, q].sum()) / max(1.0, abs(g)))
     for a in range(3):
