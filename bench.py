@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import subprocess
 import math
 import os
 import re
@@ -91,6 +92,33 @@ def roofline(s, prof_steps=50, warm_steps=50):
         "step_bytes": step_bytes,
         "step_ms_events": prof["step"],
     }
+
+
+def streaming_ceiling(nbytes, reps=200):
+    """Practical HBM streaming rate of this GPU for the field size of one pass:
+    torch's element-wise kernels over FP64 tensors of nbytes each, timed with
+    events -- mul(x, 1, out=y) moves the bytes of a line pass (one read, one
+    write), add_ those of the update (two reads, one write)."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.ones_like(x)
+    res = {}
+    for name, fn, moved in (("mul (1 read + 1 write)", lambda: torch.mul(x, 1.0, out=y), 2 * nbytes),
+                            ("add_ (2 reads + 1 write)", lambda: y.add_(x), 3 * nbytes)):
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"us": 1e3 * ms, "bytes": moved, "GBps": moved / (ms * 1e-3) / 1e9}
+    del x, y
+    torch.cuda.empty_cache()
+    return res
 
 
 def load_pmc_traffic(tag, kernel):
@@ -489,6 +517,8 @@ def main():
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="turn two rocprofv3 --pmc outputs into pmc_traffic_*.json")
     ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles"))
+    ap.add_argument("--streaming-ceiling", type=int, default=0, metavar="BYTES",
+                    help="print torch's element-wise streaming rates over FP64 tensors of BYTES each (JSON)")
     ap.add_argument("--workload", default="c1", choices=["c1", "ensemble", "adjoint", "frank-read"],
                     help="c1: the 128^3 x 12 headline (default); the others print their own JSON line")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="ensemble precision")
@@ -506,6 +536,9 @@ def main():
               "the multi-GPU slab decomposition is a design note, not built.", file=sys.stderr)
         return 2
 
+    if args.streaming_ceiling:
+        print(json.dumps(streaming_ceiling(args.streaming_ceiling)))
+        return 0
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "adjoint":
@@ -540,6 +573,24 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(prob, args.cpu_steps)
+    # the practical streaming ceiling the passes are compared with (the
+    # roofline peak stays the nominal 8 TB/s)
+    # measured in a child process: torch brings its own HIP runtime, which
+    # cannot initialise next to the one the solver library already uses here
+    # (per precision: the FP32 field is half the size, and part of it stays in
+    # the last-level cache between passes)
+    ceil = {}
+    for tag, o in out.items():
+        nb = int(o["roofline"]["per_kernel"]["fwd_z"]["bytes"]) // 2
+        child = subprocess.run([sys.executable, os.path.abspath(__file__), "--streaming-ceiling", str(nb)],
+                               capture_output=True, text=True, timeout=600)
+        if child.returncode != 0:
+            raise RuntimeError("streaming-ceiling probe failed: " + child.stderr[-2000:])
+        ceil[tag] = json.loads(child.stdout.strip().splitlines()[-1])
+        rate = {"line": ceil[tag]["mul (1 read + 1 write)"]["GBps"],
+                "update": ceil[tag]["add_ (2 reads + 1 write)"]["GBps"]}
+        for role, k in o["roofline"]["per_kernel"].items():
+            k["frac_of_streaming_ceiling"] = k["GBps"] / rate["update" if role == "inv_z_update" else "line"]
 
     f64, f32 = out["fp64"], out["fp32"]
     line = {
@@ -573,6 +624,7 @@ def main():
             "fp32": {"bytes": f32["roofline"]["step_bytes"],
                      "frac": f32["roofline"]["step_bytes"] / (f32["ms_per_step"] * 1e-3) / PEAK_HBM}},
         "per_kernel": {"fp64": f64["roofline"]["per_kernel"], "fp32": f32["roofline"]["per_kernel"]},
+        "streaming_ceiling": ceil,
         # the same K steps with the field handed over in host memory (one
         # upload before, one download after: the C ABI's set_xi / get_xi)
         "pcie_inclusive": {
