@@ -355,6 +355,50 @@ def test_ranmar_published_test_values():
     assert got == [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0]
 
 
+def _rotation(axis, ang):
+    a = np.asarray(axis, dtype=float)
+    a /= np.linalg.norm(a)
+    K = np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+    return np.eye(3) + math.sin(ang) * K + (1.0 - math.cos(ang)) * K @ K
+
+
+def test_rotated_crystal_resolved_shear_relation():
+    """Pins the rotation conventions of rotate_stress, set_scale and the
+    slip-vector rotation together, in the oracle and in the host layer.
+    From the reference's own formulas, for a rotation R given as the
+    primitive vectors (the columns of prim, fft.cpp:104-132):
+      * four = prim^-1 = R^T (the cofactor formula of set_scale, app.cpp:411-421);
+      * the slip vectors become four n, four b (app_fcc.cpp:288-313);
+      * sigma_rot = prim sigma prim^-1 = R sigma R^T (fft_fftw_slab.cpp:252-261, |det| = 1).
+    Hence tau(R, sigma) = (R^T b).(R sigma R^T)(R^T n) = tau(I, R^2 sigma R^2^T): the
+    stress acts as if rotated twice. A transposed convention in any one of
+    the four pieces gives tau(I, sigma) or tau(I, R^T^2 sigma R^2) instead,
+    which the second assertion tells apart."""
+    import pfdd_setup
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((3, 3))
+    sigma = 0.01 * (A + A.T)
+    xn0, xb0 = P.FCC_XN.copy(), P.FCC_XB.copy()
+
+    def tau_oracle(R, sig):
+        geo = P.geometry(tuple(map(tuple, R.T)))
+        xn, xb = oracle.rotate_slip(xn0, xb0, geo["four"], geo["normfour"])
+        return np.asarray(P.tau_of(geo, sig, xn, xb)[0])
+
+    def tau_host(R, sig):
+        return pfdd_setup.resolved_shear("fcc", sig, P.DSLIP, prim=R.T.reshape(9))[0]
+
+    I = np.eye(3)
+    for trial in range(4):
+        R = _rotation(rng.standard_normal(3), rng.uniform(0.2, 3.0))
+        R2 = R @ R
+        for tau in (tau_oracle, tau_host):
+            t = tau(R, sigma)
+            scale = np.max(np.abs(t))
+            assert np.max(np.abs(t - tau(I, R2 @ sigma @ R2.T))) <= 1e-13 * scale
+            assert np.max(np.abs(t - tau(I, sigma))) > 1e-3 * scale
+
+
 def test_resolss_non_schmid_closed_forms():
     """resolSS_non_Schmid (fft_fftw_slab.cpp:774-845) against the projection
     tensor written as outer products, S = b (x) n + w1 b (x) n' + w2 (n x b) (x) n
