@@ -531,7 +531,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
     constexpr int TPP = otf_tpp<R, N, S>();
     constexpr int SA = S / TPP;
     constexpr int NPT = NP * N; // k-points per tile
-    fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
+    fft_stages_x<N, 1, NL, NTHR, false>(lds, tw, tid);
     // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
     // replaces xihat_a in the tile
     double el = 0.0;
@@ -581,7 +581,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
         }
     }
     __syncthreads();
-    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
+    fft_stages_x<N, 1, NL, NTHR, true>(lds, tw, tid);
     return el;
 }
 

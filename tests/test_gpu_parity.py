@@ -82,7 +82,7 @@ def test_stored_B_fp32_and_skewed():
 
 
 @pytest.mark.parametrize("shape", [(48, 16, 32), (16, 96, 8), (8, 32, 48), (40, 16, 80), (16, 160, 8),
-                                   (192, 8, 16), (80, 40, 8)])
+                                   (192, 8, 16), (80, 40, 8), (256, 8, 8), (512, 8, 8)])
 def test_mixed_radix_grids(shape):
     _say(f"case: {shape} x 3 basal, 10 steps, FP64")
     nx, ny, nz = shape
