@@ -90,6 +90,16 @@ def test_mixed_radix_grids(shape):
     run_pair(case, 10, check_every=3)
 
 
+@pytest.mark.parametrize("shape", [(48, 16, 16), (80, 16, 8), (128, 16, 8), (256, 8, 8)])
+def test_radix16_x_lengths_fp32(shape):
+    """The x pass takes a radix-16 first stage at these lengths; FP32 mode
+    against the FP64 oracle at the FP32 tolerance."""
+    _say(f"case: {shape} x 3 basal, 10 steps, FP32 (radix-16 x pass)")
+    nx, ny, nz = shape
+    case = P.make_case(shape=shape, noise=1e-2, loops=[(1, 2, nz // 2, nx / 2, ny / 2, 3)])
+    run_pair(case, 10, precision=1, check_every=3, tol_field=1e-5, tol_scal=1e-5)
+
+
 def test_skewed_grid_masked_frank_read_like():
     _say("case: 32x32x16 skewed (60 deg) grid, mask, 30 steps, FP64")
     shape = (32, 32, 16)
