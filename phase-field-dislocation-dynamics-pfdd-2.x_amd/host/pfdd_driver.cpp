@@ -480,6 +480,16 @@ void App::init_app(FFT_HIP &fft)
     if (fft.core_flag == 2 && fft.slip_systems % 3)
         deck_error("core_energy 2 couples the slip systems of each plane in threes");
     if (fft.core_flag == 7 && fft.mat.A_mpea.empty()) deck_error("core_energy 7 needs 'material ... mpea FILE'");
+    if (fft.core_flag == 7) {
+        // Material::read_mpea_file (material.cpp:848-853): the header must match the box
+        const int *d = fft.mat.mpea_dims;
+        if (d[0] != fft.nx || d[1] != fft.ny || d[2] != fft.nz || d[3] != fft.slip_systems) {
+            char buf[256];
+            snprintf(buf, sizeof(buf), "MPEA core energy file size does not match box %d %d %d %d %d %d %d %d",
+                     fft.nx, d[0], fft.ny, d[1], fft.nz, d[2], fft.slip_systems, d[3]);
+            deck_error(buf);
+        }
+    }
     Grid g{fft.nx, fft.ny, fft.nz, fft.slip_systems};
     const long long n = (long long)g.S * g.N();
     fft.xi.assign(2 * n, 0.0);
@@ -747,6 +757,10 @@ void Input::command(const std::vector<std::string> &w)
                 int n1, n2, n3, ns;
                 if (fscanf(f, "%d %d %d %d", &n1, &n2, &n3, &ns) != 4) deck_error("bad MPEA file header");
                 m.A_mpea.assign((size_t)n1 * n2 * n3 * ns, 0.0);
+                m.mpea_dims[0] = n1;
+                m.mpea_dims[1] = n2;
+                m.mpea_dims[2] = n3;
+                m.mpea_dims[3] = ns;
                 int a, b, cc, s;
                 double v;
                 while (fscanf(f, "%d %d %d %d %lf", &a, &b, &cc, &s, &v) == 5)

@@ -30,6 +30,7 @@ struct Material {
     // Voigt re-derivation stored by Bmatrix (fft_fftw_slab.cpp:487-496)
     double mu_s = 0, ll_s = 0, young_s = 0, xnu_s = 0;
     std::vector<double> A_mpea; // per-cell USFE of core flag 7 (J/m^2), [a][i][j][k]
+    int mpea_dims[4] = {0, 0, 0, 0}; // n1 n2 n3 ns of the MPEA file header
 
     bool preset(const std::string &name); // the named presets of material.cpp
     void derive();

@@ -184,6 +184,12 @@ class GLSolver:
         self._coef = _f64(coef).reshape(-1) if coef is not None else None
         self._amp = _f64(amp).reshape(-1) if amp is not None else None
         self._xo = _f64(xo).reshape(-1) if xo is not None else None
+        # the library reads batch * S * N values from each per-cell array
+        need = self.SM * int(np.prod(self.shape))
+        for name, arr in (("amp", self._amp), ("xo", self._xo)):
+            if arr is not None and arr.size != need:
+                raise ValueError(f"{name} has {arr.size} values; a batch of {self.batch} x {self.S} systems "
+                                 f"on {self.shape} needs {need} (one per member, system and cell)")
         d = GLDesc()
         d.nx, d.ny, d.nz = self.shape
         d.ns = self.S

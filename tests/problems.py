@@ -105,9 +105,32 @@ def loop_field(shape, S, loops, noise, seed=576):
     return xi
 
 
+def amplitude_field(shape, S, seed=576, cv=0.3, same_systems=False):
+    """A smooth, positive, non-uniform per-cell amplitude (S, nx, ny, nz): the
+    exponential of a low-pass filtered Gaussian field (numpy generator), mean
+    about 1, coefficient of variation about `cv`; different on every system
+    unless `same_systems`."""
+    rng = np.random.default_rng(seed)
+    k = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
+    filt = np.exp(-(np.pi * 2.0) ** 2 * (k[0] ** 2 + k[1] ** 2 + k[2] ** 2))
+    out = np.empty((S,) + tuple(shape))
+    for a in range(S):
+        if a and same_systems:
+            out[a] = out[0]
+            continue
+        g = np.fft.ifftn(np.fft.fftn(rng.standard_normal(tuple(shape))) * filt).real
+        g = (g - g.mean()) / g.std()
+        out[a] = np.exp(cv * g - 0.5 * cv * cv)
+    return out
+
+
 def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigma_mu=0.00625,
               core="sine", cddt=0.01, prim_vectors=((1, 0, 0), (0, 1, 0), (0, 0, 1)), xo=None,
-              seed=576, ncoef=4):
+              seed=576, ncoef=4, amp=None):
+    """amp: optional per-cell amplitude (S, nx, ny, nz) of the sine-series
+    core (core "series"); core "mpea" (flag 7) takes the per-cell unstable
+    stacking-fault energy A_mpea = usf * amp (J/m^2, amp defaulting to
+    amplitude_field) and divides it by mu dslip b as core_energy_mpea does."""
     mat = material()
     geo = geometry(prim_vectors)
     if systems == "basal3":
@@ -143,6 +166,13 @@ def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigm
             par[:, 2:] = 0.1 * base[0] * (2 * rng.random((S, ncoef - 2)) - 1)
         flag, nm = 8, ncoef
         gpu_core = dict(core_model=8, coef=par.reshape(-1), ncoef=ncoef)
+    elif core == "mpea":
+        mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
+        if amp is None:
+            amp = amplitude_field(shape, S, seed)
+        amp = USF * np.asarray(amp, dtype=float) / mdb  # A_mpea / (mu dslip b)
+        flag, par, nm = 7, np.zeros(1), 0
+        gpu_core = dict(core_model=7, coef=None, ncoef=0)
     elif core == "bcc":
         mdb = mat["mu_s"] * mat["dslip"] * mat["b"]
         par = np.array([USF / mdb])
@@ -165,8 +195,12 @@ def make_case(shape=(16, 16, 16), systems="basal3", loops=None, noise=1e-3, sigm
         gpu_core = dict(core_model=0, coef=None, ncoef=0)
     else:
         raise ValueError(core)
+    if amp is not None:
+        amp = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, dtype=float), (S,) + tuple(shape)))
+        assert flag in (7, 8), "a per-cell amplitude belongs to core flags 7 and 8"
+        gpu_core = dict(gpu_core, amp=amp)
     return {
-        "shape": tuple(shape), "S": S, "mat": mat, "geo": geo, "xn": xn, "xb": xb, "xi0": xi0,
+        "shape": tuple(shape), "S": S, "mat": mat, "geo": geo, "xn": xn, "xb": xb, "xi0": xi0, "amp": amp,
         "tau": tau, "sigma": sigma, "sigma_rot": srot, "cddt": cddt, "core_flag": flag,
         "core_par": np.atleast_1d(par).reshape(-1), "nm": nm, "gpu_core": gpu_core, "xo": xo,
     }
@@ -183,7 +217,8 @@ def oracle_state(case, BB=None):
     if BB is None:
         BB, _, _ = oracle_B(case)
     return oracle.GLState(case["xi0"], BB, case["cddt"], case["core_flag"], case["core_par"],
-                          case["nm"], None, case["xo"], four=case["geo"]["four"], xn=case["xn"], xb=case["xb"])
+                          case["nm"], case.get("amp"), case["xo"], four=case["geo"]["four"], xn=case["xn"],
+                          xb=case["xb"])
 
 
 def gpu_solver(case, precision=0, stored_B=False):

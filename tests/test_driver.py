@@ -220,6 +220,19 @@ def test_deck_text_and_binary_dumps(tmp_path):
 
 
 @pytestmark_cpu
+def test_mpea_file_must_match_the_box(tmp_path):
+    """Material::read_mpea_file (material.cpp:848-853) rejects a per-cell
+    USFE file whose header does not match the box and slip systems."""
+    lines = ["8 16 16 3"] + [f"{i} {j} {k} {s} 158.0" for s in range(3) for i in range(8) for j in range(16)
+                             for k in range(16)]
+    (tmp_path / "usfe.txt").write_text("\n".join(lines) + "\n")
+    text = deck_text().replace("core_energy 3", "core_energy 7").replace("usf 158e-3", "usf 158e-3 mpea usfe.txt")
+    r = run_exe(text, tmp_path, check=True)
+    assert r.returncode == 1
+    assert "MPEA core energy file size does not match box 16 8 16 16 16 16 3 3" in r.stderr
+
+
+@pytestmark_cpu
 @pytest.mark.parametrize("preset,flag,names", [
     ("Cu", 2, [344.948e-3, -85.399e-3, -40.421e-3, 12.396e-3, -0.674e-3, -131.393e-3, -19.816e-3]),
     ("Ti", 4, [427.0e-3, -165.9e-3, -221.9e-3, -26.29e-3, -13.42e-3, -53.09e-3, 119.1e-3, -14.25e-3, -17.77e-3]),

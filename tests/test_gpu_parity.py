@@ -90,6 +90,47 @@ def test_mixed_radix_grids(shape):
     run_pair(case, 10, check_every=3)
 
 
+@pytest.mark.parametrize("shape,precision", [((64, 64, 16), 0), ((64, 64, 16), 1), ((16, 64, 64), 0),
+                                             ((16, 64, 64), 1), ((8, 512, 16), 0), ((8, 512, 16), 1),
+                                             ((8, 16, 512), 0), ((8, 16, 512), 1), ((16, 256, 8), 0),
+                                             ((8, 8, 256), 1)])
+def test_line_lengths_of_the_large_configs(shape, precision):
+    """The line lengths of the 64^3 configurations (64 on every axis) and of
+    the 512 x 512 x 256 grid (512 along y and z, 256 along y and z), which
+    the other parity shapes put on x only: every pass kernel at those
+    lengths in both precisions."""
+    _say(f"case: {shape} x 3 basal, 10 steps, precision {precision}")
+    nx, ny, nz = shape
+    case = P.make_case(shape=shape, noise=1e-2, loops=[(0, 2, nz // 2, nx / 2, ny / 2, min(nx, ny) // 3)])
+    tol = 1e-9 if precision == 0 else 1e-5
+    run_pair(case, 10, precision=precision, check_every=3, tol_field=tol, tol_scal=tol)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_skewed_masked_256_z_lines(precision):
+    """Skewed (60 deg) grid with a pinned segment, 256 cells along z: the
+    update kernel's 256-point inverse transform with the obstacle mask."""
+    _say(f"case: 16x16x256 x 3 skewed, mask, 12 steps, precision {precision}")
+    shape = (16, 16, 256)
+    xo = np.zeros((3,) + shape)
+    xo[:, 3, 7:9, 128] = 1.0
+    xo[:, 12, 7:9, 128] = 1.0
+    case = P.make_case(shape=shape, noise=1e-2, sigma_mu=0.01, xo=xo, loops=[(0, 2, 128, 8, 8, 5)],
+                       prim_vectors=((1, 0, 0), (0.5, 0.8660254037844386, 0), (0, 0, 1)))
+    tol = 1e-9 if precision == 0 else 1e-5
+    run_pair(case, 12, precision=precision, check_every=4, tol_field=tol, tol_scal=tol)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fcc12_64_point_x_pass(precision):
+    """12 systems with 64-point x lines (the x pass at its 12-system tile)."""
+    _say(f"case: 64x16x16 x 12 fcc, 8 steps, precision {precision}")
+    case = P.make_case(shape=(64, 16, 16), systems="fcc12", noise=1e-2, sigma_mu=0.004,
+                       loops=[(0, 2, 8, 32, 8, 5), (7, 0, 20, 8, 8, 4)])
+    tol = 1e-9 if precision == 0 else 1e-5
+    run_pair(case, 8, precision=precision, check_every=4, tol_field=tol, tol_scal=tol)
+
+
 @pytest.mark.parametrize("shape", [(48, 16, 16), (80, 16, 8), (128, 16, 8), (256, 8, 8)])
 def test_radix16_x_lengths_fp32(shape):
     """The x pass takes a radix-16 first stage at these lengths; FP32 mode

@@ -168,6 +168,69 @@ def test_core_energy_derivative_fd(flag):
     assert abs((ep - em) / (2 * h) - dE.sum()) < 1e-7 * np.abs(dE).sum()
 
 
+def _fd_amp(xi, flag, par, nm, amp, picks, h=1e-6):
+    """Central differences of the variational core energy with respect to
+    single elements (a, flat cell r) of Re xi."""
+    out = []
+    for (a, r) in picks:
+        xp, xm = xi.copy(), xi.copy()
+        xp.reshape(xi.shape[0], -1)[a, r] += h
+        xm.reshape(xi.shape[0], -1)[a, r] -= h
+        _, _, ep = oracle.core_energy(xp, flag, par, nm, amp)
+        _, _, em = oracle.core_energy(xm, flag, par, nm, amp)
+        out.append((ep - em) / (2 * h))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("flag", [7, 8])
+def test_core_energy_per_cell_amplitude(flag):
+    """core_energy_mpea (flag 7, fft_fftw_slab.cpp:1196-1235: per-cell
+    A_mpea[core_index], core_index = (lxs+i)*N2*N3 + j*N3 + k + isa*N1*N2*N3)
+    and the amp factor of the sine series (flag 8):
+      * the force is the derivative of the energy density: central
+        differences of single elements with a non-uniform amplitude that
+        differs between systems and cells;
+      * index map: an amplitude that is 1 except one cell of one system (3)
+        and a system with amplitude 0 reduces flag 7 to flag 1 (An' = 1)
+        times that pattern, so a transposed or system-blind amp index fails;
+      * flag 8 with one term of coefficient 1 equals flag 7 (forces and both
+        energies); flag 8 with amp = 1 equals flag 8 without amp;
+      * E_core as the reference accumulates it is the density sum / nx."""
+    rng = np.random.default_rng(11)
+    S, shape = 3, (4, 5, 6)
+    xi = (rng.random((S,) + shape) * 2 - 0.5) + 1j * rng.random((S,) + shape)
+    amp = np.exp(0.3 * rng.standard_normal((S,) + shape))
+    if flag == 7:
+        par, nm = np.zeros(1), 0
+    else:
+        par, nm = rng.random(S * 3) * 0.01, 3
+    dE, eref, ev = oracle.core_energy(xi, flag, par, nm, amp)
+    picks = [(0, 0), (1, 17), (2, 119), (2, 3), (0, 64), (1, 100)]
+    fd = _fd_amp(xi, flag, par, nm, amp, picks)
+    got = np.array([dE.reshape(S, -1)[a, r] for a, r in picks])
+    assert np.allclose(got, fd, rtol=1e-7, atol=1e-9 * np.abs(dE).max())
+    assert abs(eref - ev / shape[0]) <= 1e-14 * abs(ev)
+    # index map through a pattern amplitude, against flag 1 with An' = 1
+    pat = np.ones((S,) + shape)
+    pat[2, 1, 3, 4] = 3.0
+    pat[1] = 0.0
+    d1, _, e1 = oracle.core_energy(xi, 1, np.array([1.0]))
+    if flag == 7:
+        dp, _, ep = oracle.core_energy(xi, 7, np.zeros(1), 0, pat)
+        assert np.allclose(dp, pat * d1, rtol=1e-15, atol=0)
+        dens1 = np.sin(np.pi * xi.real) ** 2
+        assert abs(ep - np.sum(pat * dens1)) <= 1e-13 * abs(ep)
+    else:
+        one = np.ones(S)  # one term, c = 1 on every system
+        d8, r8, e8 = oracle.core_energy(xi, 8, one, 1, amp)
+        d7, r7, e7 = oracle.core_energy(xi, 7, np.zeros(1), 0, amp)
+        assert np.allclose(d8, d7, rtol=1e-14, atol=1e-15) and abs(e8 - e7) <= 1e-14 * abs(e7)
+        assert abs(r8 - r7) <= 1e-14 * abs(r7)
+        dn, _, en = oracle.core_energy(xi, 8, par, nm, None)
+        du, _, eu = oracle.core_energy(xi, 8, par, nm, np.ones((S,) + shape))
+        assert np.array_equal(dn, du) and en == eu
+
+
 def test_core_energy_sine_closed_form():
     isf, cn = 0.004, 0.013
     xi = np.array([0.0, 0.25, 0.5, 0.125]).reshape(1, 1, 1, 4).astype(complex)
@@ -467,3 +530,23 @@ def test_oracle_reproduces_committed_fixtures():
         assert int(sc[0]) == steps
         for v, w in zip((r["xinorm"], r["xiave"], r["E_total"]), sc[1:]):
             assert abs(v - w) <= 1e-12 * max(abs(w), 1.0), (name, v, w)
+
+
+def test_c0_fixture_recipe_unchanged():
+    """The committed 2000-state oracle run of the 64^3 x 3 ramp
+    (tests/golden/c0_ramp_64cubed.npz) was made from tests/problems.py's
+    seeded recipe; the recipe must still give the same initial field, and the
+    stored last stress state must be the ramp's (1999 increments on xz)."""
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_fullsize_fixtures as F
+    fx = np.load(os.path.join(here, "golden", "c0_ramp_64cubed.npz"))
+    case = F.c0_case()
+    got = [np.sum(case["xi0"].real), np.sum(np.abs(case["xi0"]) ** 2)]
+    assert np.allclose(fx["xi0_checksum"], got, rtol=1e-14, atol=0)
+    _, srot = F.tau_of_state(case, F.STATES - 1)
+    assert np.allclose(fx["sigma_rot_last"], srot, rtol=1e-15, atol=0)
+    assert abs(srot[0, 2] - (F.STATES - 1) * F.DELTA_MU) < 1e-18
+    assert fx["scalars"].shape == (F.STATES // F.EVERY, 8) and fx["scalars"][-1, 0] == F.STATES
