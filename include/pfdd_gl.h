@@ -225,14 +225,22 @@ int pfdd_gl_theta(pfdd_gl *h, double *theta);
 /* Autotuner (no counterpart in the reference): times each candidate LDS tile
  * shape (0, 1, 2: about 1K/2K/4K FP64 or 2K/4K/8K FP32 elements per tile) of
  * the four line passes -- forward z, forward y, inverse y, inverse z + update
- * -- and both x-pass variants (0: one tile per workgroup, 1: persistent
- * workgroups that load the next tile while transforming the current one)
- * on this handle's own buffers: after 20 warm-up iterations, two interleaved
- * rounds of `nsteps` GL iterations per candidate, keeping each kernel's best
- * time; the fastest per kernel wins (choice[5], may be NULL).  The field and step counters
- * are restored; the choice is cached for the process per grid, system count,
- * batch, precision and options.  Results (xi) do not depend on the choice.
- * The update of core models 2, 4 and 9 always uses tile shape 0. */
+ * -- and the iteration variant of choice[4]:
+ *   0  five kernels, x pass one tile per workgroup;
+ *   1  five kernels, x pass on persistent workgroups that load the next tile
+ *      while transforming the current one;
+ *   2  three kernels (the "plane" iteration: 2-D FFT of each y-z plane in LDS,
+ *      x pass, 2-D inverse FFT fused with the update), for square y-z planes
+ *      that fit in LDS (FP32 up to 128 x 128, FP64 up to 80 x 80), nx of
+ *      16..128, on-the-fly B.
+ * Timing runs on this handle's own buffers: after 20 warm-up iterations, two
+ * interleaved rounds of `nsteps` GL iterations per candidate, keeping each
+ * kernel's best time; the fastest shape per line pass and the faster x-pass
+ * variant win, and variant 2 replaces them if its whole iteration is faster
+ * (choice[5], may be NULL).  The field and step counters are restored; the
+ * choice is cached for the process per grid, system count, batch, precision
+ * and options.  Results (xi) do not depend on the choice.  The update of
+ * core models 2, 4 and 9 always uses tile shape 0. */
 int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5]);
 /* Sets the choices explicitly (same meaning as pfdd_gl_autotune's output). */
 int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5]);

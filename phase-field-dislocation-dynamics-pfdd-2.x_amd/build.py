@@ -23,11 +23,39 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
+# Translation units of the HIP library, compiled in parallel and linked
+# into one shared object (no device code crosses a unit boundary).
+HIP_UNITS = ("pfdd_gl.hip", "pfdd_pass.hip", "pfdd_update.hip", "pfdd_mid.hip", "pfdd_plane.hip")
+
+
 def build_lib():
-    os.makedirs(os.path.join(HERE, "lib"), exist_ok=True)
-    out = os.path.join(HERE, "lib", "libpfdd_gl.so")
-    srcs = [os.path.join(HERE, "csrc", "pfdd_gl.hip")]
-    run([HIPCC] + HIPFLAGS + ["-shared", "-o", out + ".tmp"] + srcs)
+    lib = os.path.join(HERE, "lib")
+    obj = os.path.join(lib, "obj")
+    os.makedirs(obj, exist_ok=True)
+    out = os.path.join(lib, "libpfdd_gl.so")
+    units = [u for u in HIP_UNITS if os.path.exists(os.path.join(HERE, "csrc", u))]
+    cmds, objs = [], []
+    # PFDD_INCREMENTAL=1 (development only) skips units whose object is newer
+    # than the unit and every header; the default rebuilds everything.
+    incremental = os.environ.get("PFDD_INCREMENTAL") == "1"
+    headers = [os.path.join(HERE, "csrc", f) for f in os.listdir(os.path.join(HERE, "csrc")) if f.endswith(".hpp")]
+    headers.append(os.path.join(ROOT, "include", "pfdd_gl.h"))
+    newest_hdr = max(os.path.getmtime(f) for f in headers)
+    for u in units:
+        o = os.path.join(obj, u.replace(".hip", ".o"))
+        objs.append(o)
+        src = os.path.join(HERE, "csrc", u)
+        if incremental and os.path.exists(o) and os.path.getmtime(o) > max(newest_hdr, os.path.getmtime(src)):
+            continue
+        cmds.append([HIPCC] + HIPFLAGS + ["-c", "-o", o, src])
+    procs = []
+    for c in cmds:
+        print(" ".join(c), flush=True)
+        procs.append(subprocess.Popen(c))
+    bad = [c for c, p in zip(cmds, procs) if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     os.replace(out + ".tmp", out)
     return out
 

@@ -356,6 +356,61 @@ __device__ __forceinline__ void fft_stages_x(const Tile<V> lds, const V *__restr
     }
 }
 
+// The stage of stockham_stage over a tile with arbitrary strides: element
+// (line l, position p) at l * LS + p * PS.  Lanes stay line-fastest; the
+// arithmetic (radix order, twiddle indices, butterflies) is that of
+// stockham_stage, so a line transformed here is bit-identical to the same
+// line transformed by fft_stages.  The plane kernels hold a whole y-z plane
+// in one tile and transform it along both axes, once with unit line stride
+// and once with unit position stride.
+template <int N, int RAD, int NS, int NL, int LS, int PS, int NTHR, bool INV, typename V>
+__device__ __forceinline__ void stockham_stage_g(const Tile<V> lds, const V *__restrict__ tw, int tid)
+{
+    constexpr int NBL = N / RAD;
+    constexpr int NBF = NBL * NL;
+    constexpr int PER = (NBF + NTHR - 1) / NTHR;
+    V v[PER][RAD];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int bf = tid + q * NTHR;
+        if ((NBF % NTHR == 0) || bf < NBF) {
+            const int l = bf % NL, j = bf / NL;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[q][r] = lds.ld(l * LS + (j + r * NBL) * PS);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int bf = tid + q * NTHR;
+        if ((NBF % NTHR == 0) || bf < NBF) {
+            const int l = bf % NL, j = bf / NL;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r)
+                    v[q][r] = cmulw<INV>(v[q][r], tw[r * k * (N / (NS * RAD))]);
+            }
+            dft<RAD, INV>(v[q]);
+            const int idxD = (j / NS) * NS * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) lds.st(l * LS + (idxD + r * NS) * PS, v[q][r]);
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NS, int NL, int LS, int PS, int NTHR, bool INV, typename V>
+__device__ __forceinline__ void fft_stages_g(const Tile<V> lds, const V *__restrict__ tw, int tid)
+{
+    if constexpr (NS < N) {
+        constexpr int RAD = pick_radix(N / NS);
+        static_assert(RAD == 2 || RAD == 3 || RAD == 4 || RAD == 5 || RAD == 8, "length must be 2^a 3^b 5^c");
+        stockham_stage_g<N, RAD, NS, NL, LS, PS, NTHR, INV, V>(lds, tw, tid);
+        fft_stages_g<N, NS * RAD, NL, LS, PS, NTHR, INV, V>(lds, tw, tid);
+    }
+}
+
 // Addressing of one line of a pass: line index L is split as (oa, ob, d),
 // d fastest (d < D, ob < B); element p of the line sits at
 //   oa*SA + ob*SB + d*SD + p*SP.

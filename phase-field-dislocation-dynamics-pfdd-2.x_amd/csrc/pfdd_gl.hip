@@ -38,33 +38,17 @@
 #include <string>
 #include <vector>
 
-#include "pfdd_fft.hpp"
-#include "../../include/pfdd_gl.h"
+#include "pfdd_kern.hpp"
 
 using namespace pfdd;
 
+namespace pfdd {
+thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+} // namespace pfdd
+
 namespace {
 
-constexpr int MAXS = PFDD_GL_MAX_SYSTEMS;
-constexpr int MAXM = PFDD_GL_MAX_COEF;
-constexpr int NTHR = 256;
-constexpr int NSLOT = 5; // xinorm^2, xiave, e_core, e_stress, e_el
-constexpr int NRES = 8;  // xinorm, xiave, e_el, e_core, e_stress, e_total, e_core_ref, nonfinite
-
 thread_local std::string g_err;
-
-// Kernel-timing events for the next launch of the GL iteration's kernels
-// (set by the profiler around one launch at a time): the dispatch itself
-// records them (hipExtLaunchKernelGGL), so they bracket exactly the kernel.
-thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
-
-#define PFDD_LAUNCH(kern, grid, block, st, ...)                                                        \
-    do {                                                                                              \
-        if (t_ev_start)                                                                               \
-            hipExtLaunchKernelGGL(kern, grid, block, 0, st, t_ev_start, t_ev_stop, 0, __VA_ARGS__);    \
-        else                                                                                          \
-            hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__);                                \
-    } while (0)
 
 int fail(int code, const char *fmt, ...)
 {
@@ -85,121 +69,6 @@ int fail(int code, const char *fmt, ...)
                         __FILE__, __LINE__);                                             \
     } while (0)
 
-struct StepParams {
-    double tau[MAXS];          // resolved shear per system (single-run handles)
-    const double *taud;        // device tau per system of a batch (nullptr: use tau[])
-    double coef[MAXS * MAXM];  // core coefficients per system of one member
-    double a0;                 // constant term of the Fourier core models
-    double cddt;
-    double invN;
-    long long ncell;           // cells per system
-    int model;                 // PFDD_CORE_*
-    int ncoef;
-    int S;                     // systems per member
-};
-
-// Coupled gamma surface of the three systems of one plane (core_energy_extended,
-// fft_fftw_slab.cpp:1243-1354): density f(y1, y2, y3) and its derivative with
-// respect to y1.  The surface is invariant under cyclic relabelling of the
-// three systems, so the derivative for system j is d1 of the cyclically
-// rotated triple (x_j, x_j+1, x_j+2).  c = c0..c4, a1, a3 over (mu dslip b).
-template <typename R>
-__device__ __forceinline__ void core_extended(R y1, R y2, R y3, const R *c, R &f, R &d1)
-{
-    const R arg[12] = {y1 - y2, y2 - y3, y3 - y1,                             // A1..A3
-                       2 * y1 - y2 - y3, 2 * y2 - y3 - y1, 2 * y3 - y1 - y2,  // B1..B3
-                       3 * y1 - y2 - 2 * y3, 3 * y1 - 2 * y2 - y3, 3 * y2 - y3 - 2 * y1,
-                       3 * y2 - 2 * y3 - y1, 3 * y3 - y1 - 2 * y2, 3 * y3 - 2 * y1 - y2}; // D1..D6
-    R sn[12], cs[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-        if constexpr (sizeof(R) == 8) sincospi(2.0 * (double)arg[q], (double *)&sn[q], (double *)&cs[q]);
-        else sincospif(2.0f * (float)arg[q], (float *)&sn[q], (float *)&cs[q]);
-    }
-    R s2[3], c2[3]; // angles 4 pi A
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        s2[q] = (R)2 * sn[q] * cs[q];
-        c2[q] = cs[q] * cs[q] - sn[q] * sn[q];
-    }
-    const R c0 = c[0], c1 = c[1], cc2 = c[2], c3 = c[3], c4 = c[4], a1 = c[5], a3 = c[6];
-    f = c0 + c1 * (cs[0] + cs[1] + cs[2]) + cc2 * (cs[3] + cs[4] + cs[5]) + c3 * (c2[0] + c2[1] + c2[2]) +
-        c4 * (cs[6] + cs[7] + cs[8] + cs[9] + cs[10] + cs[11]) + a1 * (sn[0] + sn[1] + sn[2]) +
-        a3 * (s2[0] + s2[1] + s2[2]);
-    d1 = (R)(2.0 * M_PI) *
-         (c1 * (sn[2] - sn[0]) + cc2 * ((R)-2 * sn[3] + sn[4] + sn[5]) + (R)2 * c3 * (s2[2] - s2[0]) +
-          c4 * ((R)-3 * sn[6] - (R)3 * sn[7] + (R)2 * sn[8] + sn[9] + sn[10] + (R)2 * sn[11]) +
-          a1 * (cs[0] - cs[2]) + (R)2 * a3 * (c2[0] - c2[2]));
-}
-
-// Reverse mode of core_extended.  The plane density is a sum of 21 terms
-// c_g T(2 pi u.x) over the plane's triple x = (x_0, x_1, x_2), T = cos for
-// the groups c1..c4 and sin for a1, a3, with the integer vectors u below (the
-// arguments A, B, D of core_extended, and 2A for c3 and a3).  For the
-// system k of the triple and mu = the triple's adjoint (masked) values:
-//   hv       = sum_i mu_i d2f/dx_k dx_i = -(2 pi)^2 sum_t c_g u_tk (u_t.mu) T_t
-//   gc[g]    = mu_k d(df/dx_k)/dc_g     = mu_k sum_{t in g} 2 pi u_tk T'_t
-// with T'_t = -sin for cosine terms and cos for sine terms (gc[0], the
-// constant c0, stays 0).
-template <typename R>
-__device__ __forceinline__ void core_extended_adj(const R x[3], const R mu[3], int k, const R *c, R &hv,
-                                                  double gc[7])
-{
-    // u vectors, coefficient index (1..6 = c1..c4, a1, a3) and type (0 cos, 1 sin)
-    constexpr signed char U[21][3] = {{1, -1, 0},  {0, 1, -1},  {-1, 0, 1},  {2, -1, -1}, {-1, 2, -1}, {-1, -1, 2},
-                                      {2, -2, 0},  {0, 2, -2},  {-2, 0, 2},  {3, -1, -2}, {3, -2, -1}, {-2, 3, -1},
-                                      {-1, 3, -2}, {-1, -2, 3}, {-2, -1, 3}, {1, -1, 0},  {0, 1, -1},  {-1, 0, 1},
-                                      {2, -2, 0},  {0, 2, -2},  {-2, 0, 2}};
-    constexpr signed char G[21] = {1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 4, 4, 4, 5, 5, 5, 6, 6, 6};
-    constexpr double tp = 2.0 * M_PI;
-    double h = 0.0;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) gc[q] = 0.0;
-#pragma unroll
-    for (int t = 0; t < 21; ++t) {
-        const R arg = (R)U[t][0] * x[0] + (R)U[t][1] * x[1] + (R)U[t][2] * x[2];
-        const R um = (R)U[t][0] * mu[0] + (R)U[t][1] * mu[1] + (R)U[t][2] * mu[2];
-        R sn, cs;
-        if constexpr (sizeof(R) == 8) sincospi(2.0 * (double)arg, (double *)&sn, (double *)&cs);
-        else sincospif(2.0f * (float)arg, (float *)&sn, (float *)&cs);
-        const bool is_sin = G[t] >= 5;
-        const double Tv = is_sin ? (double)sn : (double)cs;
-        const double Td = is_sin ? (double)cs : -(double)sn;
-        const double uk = (double)U[t][k];
-        h -= tp * tp * (double)c[G[t]] * uk * (double)um * Tv;
-        gc[G[t]] += (double)mu[k] * tp * uk * Td;
-    }
-    hv = (R)h;
-}
-
-struct BParams {
-    double sig[MAXS * 9]; // sigma_a = C : eps_a
-    double eps[MAXS * 9]; // eps_a = b_a n_a^T / dslip
-    double four[9];
-    double inv2C44;       // 1 / (2 C44)
-    double inv1mnu;       // 1 / (1 - xnu)
-    double invmu;         // 1 / mu (Voigt mu of Bmatrix)
-    int nx, ny, nz, S;
-};
-
-// Constants of the on-the-fly Green's-function contraction of the x pass:
-// per system the symmetric stress sigma_a = C:eps_a (xx,yy,zz,xy,xz,yz) and
-// eps_a folded for a symmetric contraction (xx,yy,zz, xy+yx, xz+zx, yz+zy).
-struct GParams {
-    double sig6[MAXS * 6];
-    double eps6[MAXS * 6];
-    double four[9];
-    double inv2C44, inv1mnu, invmu;
-    double C44x2, C12; // isotropic stiffness: C:Q = 2 C44 Q + C12 tr(Q) I
-    double inv_n[3];   // 1/nx, 1/ny, 1/nz (frequencies by multiplication in the x pass)
-    int nx, ny, nz;
-};
-
-__device__ __forceinline__ double frac_freq(int kk, int n)
-{
-    // frec(), fft_fftw_slab.cpp:294-322
-    return kk >= n / 2 ? (double)(kk - n) / (double)n : (double)kk / (double)n;
-}
 
 // ---------------------------------------------------------------- B(k) build
 // Packed symmetric B in the point order of T2 (index (k*ny + j)*nx + i), pair
@@ -244,630 +113,6 @@ __global__ void __launch_bounds__(NTHR) k_bbuild(RB *__restrict__ B, BParams P, 
         }
 }
 
-// ------------------------------------------------- generic FFT pass (1, 2, 4)
-template <typename R, int N, bool INV, int TQ>
-__global__ void __launch_bounds__(NTHR)
-    k_fft_pass(const typename cvec<R>::type *__restrict__ in, typename cvec<R>::type *__restrict__ out,
-               LineMap mi, LineMap mo, long long nlines, const typename cvec<R>::type *__restrict__ tw,
-               const int *__restrict__ done, const uint8_t *__restrict__ zmask)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ long long bin[NL], bout[NL];
-    const int tid = threadIdx.x;
-    const long long L0 = (long long)blockIdx.x * NL;
-    for (int l = tid; l < NL; l += NTHR) {
-        const long long L = L0 + l;
-        const bool ok = L < nlines;
-        bin[l] = ok ? line_base(mi, L) : -1;
-        bout[l] = ok ? line_base(mo, L) : -1;
-    }
-    __syncthreads();
-    tile_load<N, NL, NTHR>(lds, in, bin, mi.SP, tid);
-    __syncthreads();
-    if (zmask) {
-        // input elements whose zmask byte is set enter the transform as 0
-        for (int e = tid; e < TOT; e += NTHR) {
-            const int l = e / N, p = e % N;
-            const long long b = bin[l];
-            if (b >= 0 && zmask[b + p * mi.SP]) lds.st(p * LDP + l, mk<V>((R)0, (R)0));
-        }
-        __syncthreads();
-    }
-    fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
-    tile_store<N, NL, NTHR>(out, lds, bout, mo.SP, tid);
-}
-
-// Load the S lines of NP consecutive columns (T2[a][c0+q][0..N)) into the
-// tile, lines ordered (a, q); column-contiguous runs of N elements.
-template <int N, int NL, int NP, int NTHR, typename V>
-__device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__restrict__ T2, long long c0, long long ncol,
-                                               long long ntot, int tid)
-{
-    constexpr int LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
-    if constexpr (sizeof(V) >= 16) {
-#pragma unroll 4
-        for (int e = tid; e < TOT; e += NTHR) {
-            const int l = e / N, p = e % N;
-            const int a = l / NP, q = l % NP;
-            if (c0 + q < ncol) lds.st(p * LDP + l, ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]));
-        }
-    } else {
-        V r[PER];
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            const int e = tid + t * NTHR;
-            const int l = e / N, p = e % N;
-            const int a = l / NP, q = l % NP;
-            if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]);
-        }
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            const int e = tid + t * NTHR;
-            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[t]);
-        }
-    }
-}
-
-// --------------------------------------------- pass 3: x FFT + contraction
-// Columns per x-pass tile: the largest power of two whose S*NP lines of N
-// elements stay within `target` elements.  FP64 with 12 systems takes twice
-// the tile of the other FP64 cases: at 128^3 x 12 two columns per block (one
-// thread per k-point, no duplicated contraction work, 3 blocks per CU) ran
-// the pass 12 % faster than one column (two threads per k-point, 5 blocks).
-template <typename R, int N, int S> constexpr int mid_pairs()
-{
-    constexpr int target = sizeof(R) == 8 ? (S == 12 ? 4096 : 2048) : 4096;
-    int np = 1;
-    while (np * 2 * S * N <= target) np *= 2;
-    // the padded tile stays within 60 KB of LDS (the constants and the
-    // reduction scratch of the pass come on top)
-    while (np > 1 && (long long)N * (S * np + 1) * 2 * (int)sizeof(R) > 61440) np /= 2;
-    return np;
-}
-
-template <typename R, typename RB, int N, int S>
-__global__ void __launch_bounds__(NTHR)
-    k_mid(typename cvec<R>::type *__restrict__ T2, const RB *__restrict__ B, long long ncol,
-          long long ntot, int bpm, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
-          const int *__restrict__ done)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
-    constexpr int NPAIR = S * (S + 1) / 2;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ double red[NTHR / 64];
-    const int tid = threadIdx.x;
-    // member of a batch (bpm blocks per member), first column of the block
-    const int mem = blockIdx.x / bpm;
-    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
-    T2 += (long long)mem * S * ntot;
-    // load: lines (a, q) -> T2[a][c0+q][0..N)
-#pragma unroll 4
-    for (int e = tid; e < TOT; e += NTHR) {
-        const int l = e / N, p = e % N;
-        const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) lds.st(p * LDP + l, ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]));
-    }
-    __syncthreads();
-    fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
-    double el = 0.0;
-    for (int e = tid; e < NP * N; e += NTHR) {
-        const int q = e / N, p = e % N;
-        if (c0 + q >= ncol) continue;
-        const long long kp = (c0 + q) * N + p;
-        V x[S];
-#pragma unroll
-        for (int b = 0; b < S; ++b) x[b] = lds.ld(p * LDP + b * NP + q);
-        R g[S][2];
-#pragma unroll
-        for (int a = 0; a < S; ++a) g[a][0] = g[a][1] = (R)0;
-        int slot = 0;
-#pragma unroll
-        for (int a = 0; a < S; ++a) {
-#pragma unroll
-            for (int b = a; b < S; ++b, ++slot) {
-                const R bab = (R)B[(long long)slot * ntot + kp];
-                g[a][0] += bab * x[b].x;
-                g[a][1] += bab * x[b].y;
-                if (b != a) {
-                    g[b][0] += bab * x[a].x;
-                    g[b][1] += bab * x[a].y;
-                }
-            }
-        }
-        (void)NPAIR;
-#pragma unroll
-        for (int a = 0; a < S; ++a) {
-            el += (double)x[a].x * (double)g[a][0] + (double)x[a].y * (double)g[a][1];
-            lds.st(p * LDP + a * NP + q, mk<V>(g[a][0], g[a][1]));
-        }
-    }
-    __syncthreads();
-    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
-#pragma unroll 4
-    for (int e = tid; e < TOT; e += NTHR) {
-        const int l = e / N, p = e % N;
-        const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
-    }
-    double v[1] = {el};
-    block_sum<1, NTHR>(v, red, tid);
-    if (tid == 0) part_el[blockIdx.x] = v[0];
-}
-
-
-// x pass with B(k) evaluated on the fly in FP64 instead of streamed from HBM.
-// Bmatrix's kernel (fft_fftw_slab.cpp:577-634) is mu B_ab = eps_a:A:eps_b with
-// A = C - (C f) G (f C), G = (2 I/f^2 - f f/((1-nu) f^4))/(2 C44).  Applied as
-// internal_energy does (:1670-1695) to all systems at once:
-//   Sig = sum_b xihat_b sigma_b        (sigma_b = C:eps_b)
-//   w = Sig f,  V = f.w
-//   Q = [2 sym(w f)/f^2 - V f f/((1-nu) f^4)] / (2 C44)
-//   P = Sig - C:Q = Sig - 2 C44 Q - C12 tr(Q) I
-//   ghat_a = eps_a : P / mu
-// (sigma_a:Q = eps_a:(C:Q) for the isotropic C), and ghat = 0 at f = 0
-// (B(0) = 0).  TPP threads share one k-point (wave-uniform split of the
-// systems), each producing S/TPP of the outputs.
-template <typename R, int N, int S> constexpr int otf_tpp()
-{
-    constexpr int pts = mid_pairs<R, N, S>() * N;
-    return (2 * pts == NTHR && S % 2 == 0) ? 2 : 1;
-}
-
-// Scalars of the contraction in the arithmetic type T of the x pass.
-template <typename T> struct GConst {
-    T inv2C44, inv1mnu, invmu, C44x2, C12;
-};
-
-// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im),
-// computed in T (double in the FP64 path, float in the FP32 mode).
-template <typename T, int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
-__device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const T f[3], T f2,
-                                          const T *__restrict__ cst, const GConst<T> &G, Sink sink)
-{
-    // compiler-only barriers keep the (LDS-broadcast) constants from being
-    // hoisted and held in registers across points and systems
-    __asm__ volatile("" ::: "memory");
-    T Pr[6] = {}, Pi[6] = {}; // Sig, then P
-#pragma unroll
-    for (int b = 0; b < S; ++b) {
-        const V v = lds.ld(p * LDP + b * NP + q);
-        const T xr = (T)v.x, xi = (T)v.y;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            Pr[c] += cst[b * 12 + c] * xr;
-            Pi[c] += cst[b * 12 + c] * xi;
-        }
-    }
-    if (f2 > 0) {
-        // symmetric components xx, yy, zz, xy, xz, yz
-        const T wr[3] = {Pr[0] * f[0] + Pr[3] * f[1] + Pr[4] * f[2], Pr[3] * f[0] + Pr[1] * f[1] + Pr[5] * f[2],
-                              Pr[4] * f[0] + Pr[5] * f[1] + Pr[2] * f[2]};
-        const T wi[3] = {Pi[0] * f[0] + Pi[3] * f[1] + Pi[4] * f[2], Pi[3] * f[0] + Pi[1] * f[1] + Pi[5] * f[2],
-                              Pi[4] * f[0] + Pi[5] * f[1] + Pi[2] * f[2]};
-        const T Vr = f[0] * wr[0] + f[1] * wr[1] + f[2] * wr[2];
-        const T Vi = f[0] * wi[0] + f[1] * wi[1] + f[2] * wi[2];
-        const T if2 = (T)1 / f2;
-        const T c2 = if2 * G.inv2C44;                 // multiplies (w_i f_j + w_j f_i)
-        const T c4 = G.inv1mnu * if2 * if2 * G.inv2C44; // multiplies V f_i f_j
-        const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
-        T Qr[6], Qi[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const T ff = f[ci[c]] * f[cj[c]];
-            Qr[c] = c2 * (wr[ci[c]] * f[cj[c]] + wr[cj[c]] * f[ci[c]]) - c4 * Vr * ff;
-            Qi[c] = c2 * (wi[ci[c]] * f[cj[c]] + wi[cj[c]] * f[ci[c]]) - c4 * Vi * ff;
-        }
-        const T tr_r = G.C12 * (Qr[0] + Qr[1] + Qr[2]), tr_i = G.C12 * (Qi[0] + Qi[1] + Qi[2]);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            Pr[c] -= G.C44x2 * Qr[c] + (c < 3 ? tr_r : (T)0);
-            Pi[c] -= G.C44x2 * Qi[c] + (c < 3 ? tr_i : (T)0);
-        }
-    }
-#pragma unroll
-    for (int aa = 0; aa < SA; ++aa) {
-        __asm__ volatile("" ::: "memory");
-        const int a = H * SA + aa;
-        const T *ep = cst + a * 12 + 6;
-        T gr = 0, gi = 0;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            gr += Pr[c] * ep[c];
-            gi += Pi[c] * ep[c];
-        }
-        gr *= G.invmu;
-        gi *= G.invmu;
-        if (!(f2 > 0)) gr = gi = (T)0;
-        sink(aa, gr, gi);
-    }
-}
-
-// f = four^T k~ of k-point (p, col) (frec, fft_fftw_slab.cpp:280-367), formed
-// in FP64 and rounded once to T.
-template <int N, int NP, int S, typename T>
-__device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p, T f[3], T &f2)
-{
-    const int c = (int)col; // column (k, j) of one member: ny * nz < 2^31
-    const int k = c / G.ny, j = c - k * G.ny;
-    // frec's k/n with the reciprocal of n (one rounding more than the
-    // division for sizes that are not powers of two): the x pass evaluates it
-    // for every k-point of every step
-    auto fr = [](int kk, int n, double inv) { return (double)(kk >= n / 2 ? kk - n : kk) * inv; };
-    const double kx = fr(p, G.nx, G.inv_n[0]), ky = fr(j, G.ny, G.inv_n[1]), kz = fr(k, G.nz, G.inv_n[2]);
-    double fd[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) fd[c] = kx * G.four[c] + ky * G.four[3 + c] + kz * G.four[6 + c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[c] = (T)fd[c];
-    f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
-}
-
-// Minimum waves per SIMD asked of the x pass: 5 for the 12-system FP64 cases
-// (their LDS tile allows 5 blocks per CU, and the pass is latency-bound; the
-// register budget of 96 then holds without spills), no constraint otherwise.
-template <typename R, int N, int S> constexpr int otf_waves()
-{
-    return (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) ? 5 : 1;
-}
-
-// The transform of one x-pass tile held in LDS (NP columns x S systems):
-// forward FFT along x, on-the-fly contraction ghat_a = sum_b B_ab xihat_b,
-// inverse FFT.  Returns this thread's share of sum Re(conj(xihat_a) ghat_a).
-template <typename R, int N, int S>
-__device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> lds,
-                                           const typename cvec<R>::type *__restrict__ tw, const GParams &G,
-                                           const GConst<R> &gc, const R *cst, long long c0, long long ncol, int tid)
-{
-    typedef typename cvec<R>::type V;
-    typedef R T;
-    constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1;
-    constexpr int TPP = otf_tpp<R, N, S>();
-    constexpr int SA = S / TPP;
-    constexpr int NPT = NP * N; // k-points per tile
-    fft_stages_x<N, 1, NL, NTHR, false>(lds, tw, tid);
-    // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
-    // replaces xihat_a in the tile
-    double el = 0.0;
-    if constexpr (TPP == 1) {
-        // one thread per k-point: it reads all xihat_b of the point, then
-        // overwrites them with ghat_a
-        for (int pt = tid; pt < NPT; pt += NTHR) {
-            const int q = pt / N, p = pt % N;
-            if (c0 + q >= ncol) continue;
-            T f[3], f2;
-            otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
-            otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, gc, [&](int a, T gr, T gi) {
-                const int idx = p * LDP + a * NP + q;
-                const V x = lds.ld(idx);
-                el += (double)x.x * gr + (double)x.y * gi;
-                lds.st(idx, mk<V>((R)gr, (R)gi));
-            });
-        }
-    } else {
-        // two waves per set of k-points, each producing half of the systems;
-        // all reads of the tile finish (barrier) before any thread writes
-        const int h = tid / NPT, pt = tid % NPT;
-        const int q = pt / N, p = pt % N;
-        const bool ok = c0 + q < ncol;
-        T f[3], f2;
-        otf_freq<N, NP, S>(G, c0 + q, p, f, f2);
-        T g[SA][2];
-        auto keep = [&](int aa, T gr, T gi) {
-            g[aa][0] = gr;
-            g[aa][1] = gi;
-        };
-        if (ok) {
-            if (__builtin_amdgcn_readfirstlane(h) == 0)
-                otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, f2, cst, gc, keep);
-            else
-                otf_point<T, S, SA, 1, NP, LDP>(lds, p, q, f, f2, cst, gc, keep);
-        }
-        __syncthreads();
-        if (ok) {
-#pragma unroll
-            for (int aa = 0; aa < SA; ++aa) {
-                const int idx = p * LDP + (h * SA + aa) * NP + q;
-                const V x = lds.ld(idx);
-                el += (double)x.x * g[aa][0] + (double)x.y * g[aa][1];
-                lds.st(idx, mk<V>((R)g[aa][0], (R)g[aa][1]));
-            }
-        }
-    }
-    __syncthreads();
-    fft_stages_x<N, 1, NL, NTHR, true>(lds, tw, tid);
-    return el;
-}
-
-template <typename R, int S>
-__device__ __forceinline__ void otf_consts(R *cst, GConst<R> &gc, const GParams &G, int tid)
-{
-    for (int c = tid; c < S * 12; c += NTHR) cst[c] = (R)((c % 12) < 6 ? G.sig6[(c / 12) * 6 + c % 12]
-                                                                       : G.eps6[(c / 12) * 6 + c % 12 - 6]);
-    gc = GConst<R>{(R)G.inv2C44, (R)G.inv1mnu, (R)G.invmu, (R)G.C44x2, (R)G.C12};
-}
-
-// x pass, one tile per block.
-template <typename R, int N, int S>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_waves<R, N, S>(), 8)))
-    k_mid_otf(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
-              const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
-              const int *__restrict__ done)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ R cst[S * 12];
-    __shared__ double red[NTHR / 64];
-    const int tid = threadIdx.x;
-    // member of a batch (bpm blocks per member), first column of the block
-    const int mem = blockIdx.x / bpm;
-    const long long c0 = (long long)(blockIdx.x - mem * bpm) * NP;
-    T2 += (long long)mem * S * ntot;
-    GConst<R> gc;
-    otf_consts<R, S>(cst, gc, G, tid);
-    tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
-    __syncthreads();
-    double v[1] = {otf_tile<R, N, S>(lds, tw, G, gc, cst, c0, ncol, tid)};
-#pragma unroll 4
-    for (int e = tid; e < TOT; e += NTHR) {
-        const int l = e / N, p = e % N;
-        const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
-    }
-    block_sum<1, NTHR>(v, red, tid);
-    if (tid == 0) part_el[blockIdx.x] = v[0];
-}
-
-// x pass, persistent and software-pipelined: each block walks the tiles
-// t = blockIdx.x, + gridDim.x, ... and keeps the next tile's input in
-// registers, loaded while the current tile is transformed, so that memory
-// traffic and the FFT/contraction arithmetic of the resident blocks overlap
-// instead of running in lock-step phases.  Tile t is member t / bpm, columns
-// (t % bpm) * NP ...; part_el gets one partial per block.
-template <int N, int NL, int NP, int NTHR, typename V>
-__device__ __forceinline__ void cols_fetch(V *r, const V *__restrict__ T2, long long c0, long long ncol,
-                                           long long ntot, int tid)
-{
-    constexpr int TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-        const int e = tid + t * NTHR;
-        const int l = e / N, p = e % N;
-        const int a = l / NP, q = l % NP;
-        if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol) r[t] = ld_stream(&T2[(long long)a * ntot + (c0 + q) * N + p]);
-    }
-}
-
-template <typename R, int N, int S>
-__global__ void __launch_bounds__(NTHR)
-    k_mid_otf_pipe(typename cvec<R>::type *__restrict__ T2, GParams G, long long ncol, long long ntot, int bpm,
-                   int ntiles, const typename cvec<R>::type *__restrict__ tw, double *__restrict__ part_el,
-                   const int *__restrict__ done)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ R cst[S * 12];
-    __shared__ double red[NTHR / 64];
-    GConst<R> gc;
-    otf_consts<R, S>(cst, gc, G, threadIdx.x);
-    int t = blockIdx.x;
-    V r[PER];
-    {
-        const int mem = t / bpm;
-        cols_fetch<N, NL, NP, NTHR>(r, T2 + (long long)mem * S * ntot, (long long)(t - mem * bpm) * NP, ncol, ntot,
-                                    threadIdx.x);
-    }
-    double el = 0.0;
-    for (; t < ntiles; t += gridDim.x) {
-        // the thread index is re-read per tile so that per-element addresses
-        // are recomputed rather than held in registers across the loop
-        int tid = threadIdx.x;
-        __asm__ volatile("" : "+v"(tid));
-        const int mem = t / bpm;
-        const long long c0 = (long long)(t - mem * bpm) * NP;
-        V *T2m = T2 + (long long)mem * S * ntot;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int e = tid + k * NTHR;
-            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[k]);
-        }
-        __syncthreads();
-        const int tn = t + gridDim.x;
-        if (tn < ntiles) {
-            const int mn = tn / bpm;
-            cols_fetch<N, NL, NP, NTHR>(r, T2 + (long long)mn * S * ntot, (long long)(tn - mn * bpm) * NP, ncol,
-                                        ntot, tid);
-        }
-        el += otf_tile<R, N, S>(lds, tw, G, gc, cst, c0, ncol, tid);
-#pragma unroll 4
-        for (int e = tid; e < TOT; e += NTHR) {
-            const int l = e / N, p = e % N;
-            const int a = l / NP, q = l % NP;
-            if (c0 + q < ncol) T2m[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
-        }
-        __syncthreads();
-    }
-    double v[1] = {el};
-    block_sum<1, NTHR>(v, red, threadIdx.x);
-    if (threadIdx.x == 0) part_el[blockIdx.x] = v[0];
-}
-
-// ------------------------------------- pass 5: inverse z FFT + fused update
-// update_order_parameter (fft_fftw_slab.cpp:1746-1790) with the core-energy
-// derivative of core_energy_* evaluated on the old field, as the reference
-// does (it reads data_fftw = xi before the update).  Masked cells keep their
-// value; their xinorm term uses the previous unmasked cell's old value in
-// (a, i, j, k) order (prev[] precomputed on the host, -1 = the initial 0).
-template <typename R, int N, int TQ, bool GEN>
-__global__ void __launch_bounds__(NTHR)
-    k_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ xo,
-             typename cvec<R>::type *__restrict__ xn, const uint8_t *__restrict__ mask,
-             const long long *__restrict__ prev, const R *__restrict__ amp, LineMap mi, LineMap mo,
-             long long nlines, long long lines_per_sys, StepParams P,
-             const typename cvec<R>::type *__restrict__ tw, double *__restrict__ partial,
-             long long pstride, const int *__restrict__ done)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ long long bin[NL], bout[NL];
-    // per-line tau and core coefficients: read once per block here, since
-    // kernel-argument arrays indexed per lane compile to per-lane memory loads
-    __shared__ R ltau[NL], lcoef[MAXM * NL];
-    __shared__ int lsub[NL]; // position of the line's system in its plane triple (coupled model)
-    __shared__ double red[4 * (NTHR / 64)];
-    const int tid = threadIdx.x;
-    const long long L0 = (long long)blockIdx.x * NL;
-    for (int l = tid; l < NL; l += NTHR) {
-        const long long L = L0 + l;
-        const bool ok = L < nlines;
-        bin[l] = ok ? line_base(mi, L) : -1;
-        bout[l] = ok ? line_base(mo, L) : -1;
-        const int a = ok ? (int)(L / lines_per_sys) : 0; // system in the batch
-        const int as = a % P.S;                           // system within its member
-        ltau[l] = (R)(P.taud ? P.taud[a] : P.tau[a]);
-        for (int m = 0; m < MAXM; ++m) lcoef[m * NL + l] = (R)P.coef[as * MAXM + m];
-        lsub[l] = as % 3;
-    }
-    __syncthreads();
-    tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
-    // the old field of this thread's output cells, all loads in flight at
-    // once: 8-byte elements while the FFT runs, 16-byte elements right after
-    // it (holding them in registers across the FFT measured slower)
-    constexpr int PER = (TOT + NTHR - 1) / NTHR;
-    constexpr bool PREF = sizeof(V) < 16;
-    V xold[PER];
-    if constexpr (PREF) {
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            const int e = tid + t * NTHR;
-            const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
-            xold[t] = b >= 0 ? ld_stream(&xo[b + e % N]) : mk<V>((R)0, (R)0);
-        }
-    }
-    __syncthreads();
-    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
-    if constexpr (!PREF) {
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            const int e = tid + t * NTHR;
-            const long long b = (TOT % NTHR == 0 || e < TOT) ? bout[e / N] : -1;
-            xold[t] = b >= 0 ? ld_stream(&xo[b + e % N]) : mk<V>((R)0, (R)0);
-        }
-    }
-
-    const R cddt = (R)P.cddt, invN = (R)P.invN;
-    const int nm = P.ncoef;
-    double s_norm = 0.0, s_ave = 0.0, s_core = 0.0, s_str = 0.0;
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-        const int e = tid + t * NTHR;
-        if (TOT % NTHR != 0 && e >= TOT) continue;
-        const int l = e / N, p = e % N; // mo.SP == 1: rows of xi are contiguous
-        const long long b = bout[l];
-        if (b < 0) continue;
-        const long long idx = b + p;
-        const V g = lds.ld(p * LDP + l);
-        V x;
-        x = xold[t];
-        // core energy density and its derivative at Re xi of the old field
-        R dens = (R)0, dE = (R)0;
-        if (GEN && P.model == PFDD_CORE_EXTENDED) {
-            // the other two systems of the plane at the same cell
-            const int j = lsub[l];
-            const long long b0 = idx - (long long)j * P.ncell;
-            R y[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) y[q] = q == j ? x.x : xo[b0 + q * P.ncell].x;
-            R cc[7];
-#pragma unroll
-            for (int q = 0; q < 7; ++q) cc[q] = lcoef[q * NL + l];
-            R f;
-            core_extended<R>(y[j], y[(j + 1) % 3], y[(j + 2) % 3], cc, f, dE);
-            dens = f / (R)3; // each of the three systems carries a third of the plane's density
-        } else {
-            R sp, cp;
-            if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
-            else sincospif((float)x.x, (float *)&sp, (float *)&cp);
-            if (GEN && (P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER)) {
-                // a0 + sum_m a_m cos(2 pi m x) + b_m sin(2 pi m x), m <= 4
-                const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
-                R sm = s1, cm = c1;
-                dens = (R)P.a0;
-#pragma unroll
-                for (int m = 1; m <= 4; ++m) {
-                    const R am = lcoef[(m - 1) * NL + l], bm = lcoef[(m + 3) * NL + l];
-                    dens += am * cm + bm * sm;
-                    dE += (R)(2.0 * M_PI * m) * (bm * cm - am * sm);
-                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
-                    sm = sn;
-                    cm = cn;
-                }
-                // core_energy_pyrII as executed: its dE_core assignment follows
-                // the cell loop (fft_fftw_slab.cpp:1491), so only each system's
-                // last cell gets the force
-                if (P.model == PFDD_CORE_PYRII && (idx % P.ncell) != P.ncell - 1) dE = (R)0;
-            } else {
-                // sine series sum_m c_m sin^2(m pi x) (flags 1, 3, 7, 8)
-                R sm = sp, cm = cp;
-                for (int m = 1; m <= nm; ++m) {
-                    const R c = lcoef[(m - 1) * NL + l];
-                    dens += c * sm * sm;
-                    dE += c * ((R)m * (R)M_PI) * ((R)2 * sm * cm);
-                    const R sn = sm * cp + cm * sp, cn = cm * cp - sm * sp;
-                    sm = sn;
-                    cm = cn;
-                }
-            }
-            R A = (R)1;
-            if (amp) A = (R)amp[idx];
-            dens *= A;
-            dE *= A;
-        }
-        const R tau = ltau[l];
-        V y;
-        double dn;
-        if (mask && mask[idx]) {
-            y = x;
-            V pv = mk<V>((R)0, (R)0);
-            const long long pi = prev[idx];
-            if (pi >= 0) pv = xo[pi];
-            const double dr = (double)x.x - (double)pv.x, di = (double)x.y - (double)pv.y;
-            dn = dr * dr + di * di;
-        } else {
-            y.x = x.x - cddt * (g.x * invN - tau + dE);
-            y.y = x.y - cddt * (g.y * invN);
-            const double dr = (double)y.x - (double)x.x, di = (double)y.y - (double)x.y;
-            dn = dr * dr + di * di;
-        }
-        st_stream(&xn[idx], y);
-        s_norm += dn;
-        s_ave += (double)y.x;
-        s_core += (double)dens;
-        s_str += (double)tau * (double)x.x;
-    }
-    double v[4] = {s_norm, s_ave, s_core, s_str};
-    block_sum<4, NTHR>(v, red, tid);
-    if (tid == 0) {
-        partial[0 * pstride + blockIdx.x] = v[0];
-        partial[1 * pstride + blockIdx.x] = v[1];
-        partial[2 * pstride + blockIdx.x] = v[2];
-        partial[3 * pstride + blockIdx.x] = v[3];
-    }
-}
 
 // ------------------------------------- character angle (core flag 5)
 // gradient() (fft_fftw_slab.cpp:869-1065) and the angle factor of
@@ -918,140 +163,6 @@ __global__ void __launch_bounds__(NTHR)
         c = th < 1.23 ? 1.1603 * th * th - 2.0431 * th + 1 : 0.5473 * th * th - 2.0035 * th + 1.8923;
     }
     cof[(long long)a * N + r] = (R)(THETA ? th : c);
-}
-
-// ------------------------------------------ adjoint of pass 5 (reverse mode)
-// One GL iteration (k_update) maps xi to
-//   xi' = xi - cddt U (g/N - tau + dE(Re xi))      (Re part; Im: xi' = xi - cddt U g/N)
-// with U = 1 on unmasked cells and g = IFFT(B FFT xi).  The elastic operator
-// is self-adjoint (B(k) real, symmetric in (a, b) and even in k), so for
-// lam' = dL/dxi' (Re and Im taken as independent real variables)
-//   mu  = U lam'
-//   lam = lam' - cddt (IFFT(B FFT mu)/N + dE'(Re xi) Re mu)   (dE' acts on Re only)
-//   dL/dtau_a   += cddt sum_r Re mu_a
-//   dL/dc_{a,m} += -cddt sum_r Re mu_a A m pi sin(2 m pi Re xi_a)
-// with dE = A sum_m c_m m pi sin(2 m pi x), dE' = A sum_m c_m 2 m^2 pi^2 cos(2 m pi x).
-// The extended model couples the three systems of a plane: there dE'(Re xi)
-// Re mu becomes the Hessian of the plane density applied to the triple's
-// Re mu (core_extended_adj).
-// T1 holds mu after passes 1-4; this kernel applies the last inverse pass and
-// the update above, and leaves per-block sums of Re mu (slot 0) and of
-// Re mu A m pi sin(2 m pi x) (slot m) in gpart.  Every block covers lines of
-// one system (the host checks nx*ny % NL == 0).
-template <typename R, int N, int TQ>
-__global__ void __launch_bounds__(NTHR)
-    k_adj_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ lam,
-                 const typename cvec<R>::type *__restrict__ xs, typename cvec<R>::type *__restrict__ lam_out,
-                 const uint8_t *__restrict__ mask, const R *__restrict__ amp, LineMap mi, LineMap mo,
-                 long long nlines, long long lines_per_sys, StepParams P,
-                 const typename cvec<R>::type *__restrict__ tw, double *__restrict__ gpart, long long gstride,
-                 const int *__restrict__ done)
-{
-    typedef typename cvec<R>::type V;
-    if (*done) return;
-    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
-    constexpr int PER = (TOT + NTHR - 1) / NTHR;
-    constexpr int NG = 1 + MAXM;
-    PFDD_TILE(V, lds, N * LDP);
-    __shared__ long long bin[NL], bout[NL];
-    __shared__ double red[NG * (NTHR / 64)];
-    const int tid = threadIdx.x;
-    const long long L0 = (long long)blockIdx.x * NL;
-    for (int l = tid; l < NL; l += NTHR) {
-        const long long L = L0 + l;
-        const bool ok = L < nlines;
-        bin[l] = ok ? line_base(mi, L) : -1;
-        bout[l] = ok ? line_base(mo, L) : -1;
-    }
-    __syncthreads();
-    tile_load<N, NL, NTHR>(lds, T1, bin, mi.SP, tid);
-    __syncthreads();
-    fft_stages<N, 1, NL, NTHR, true>(lds, tw, tid);
-
-    const int a = (int)(L0 / lines_per_sys) % P.S;
-    const R cddt = (R)P.cddt, invN = (R)P.invN;
-    const int nm = P.ncoef;
-    double acc[NG];
-#pragma unroll
-    for (int q = 0; q < NG; ++q) acc[q] = 0.0;
-#pragma unroll 2
-    for (int t = 0; t < PER; ++t) {
-        const int e = tid + t * NTHR;
-        if (TOT % NTHR != 0 && e >= TOT) continue;
-        const int l = e / N, p = e % N;
-        const long long b = bout[l];
-        if (b < 0) continue;
-        const long long idx = b + p;
-        const V g = lds.ld(p * LDP + l);
-        const V lm = lam[idx];
-        const V x = xs[idx];
-        const R mr = (mask && mask[idx]) ? (R)0 : lm.x;
-        const R A = amp ? (R)amp[idx] : (R)1;
-        // sin, cos of 2 pi m x by the angle-addition recurrence from m = 1
-        R sp, cp;
-        if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
-        else sincospif((float)x.x, (float *)&sp, (float *)&cp);
-        const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
-        R sm = s1, cm = c1, d2 = (R)0, hv = (R)0;
-        if (P.model == PFDD_CORE_EXTENDED) {
-            // the triple of systems of this plane at the same cell (no amp[])
-            const int j = a % 3;
-            const long long b0 = idx - (long long)j * P.ncell;
-            R y[3], m3[3], cc[7];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const long long iq = b0 + q * P.ncell;
-                y[q] = q == j ? x.x : xs[iq].x;
-                m3[q] = q == j ? mr : ((mask && mask[iq]) ? (R)0 : lam[iq].x);
-            }
-#pragma unroll
-            for (int q = 0; q < 7; ++q) cc[q] = (R)P.coef[a * MAXM + q];
-            double gc[7];
-            core_extended_adj<R>(y, m3, j, cc, hv, gc);
-#pragma unroll
-            for (int q = 0; q < 7; ++q) acc[1 + q] += gc[q];
-        } else if (P.model == PFDD_CORE_FOURIER || P.model == PFDD_CORE_PYRII) {
-            // Fourier density a0 + sum_m a_m cos(w x) + b_m sin(w x), w = 2 pi m:
-            //   dE = sum_m w (b_m cos - a_m sin),  dE' = -sum_m w^2 (a_m cos + b_m sin)
-            //   d(dE)/d a_m = -w sin(w x),  d(dE)/d b_m = w cos(w x)   (slots m, 4 + m)
-            // pyrII as executed: only the last cell of each system carries dE
-            if (P.model == PFDD_CORE_FOURIER || (idx % P.ncell) == P.ncell - 1) {
-#pragma unroll
-                for (int m = 1; m <= 4; ++m) {
-                    const R am = (R)P.coef[a * MAXM + m - 1], bm = (R)P.coef[a * MAXM + m + 3];
-                    const double w = 2.0 * M_PI * m;
-                    d2 -= (R)(w * w) * (am * cm + bm * sm);
-                    acc[m] += (double)mr * (double)A * (-w * (double)sm);
-                    acc[4 + m] += (double)mr * (double)A * (w * (double)cm);
-                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
-                    sm = sn;
-                    cm = cn;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int m = 1; m <= MAXM; ++m) {
-                if (m <= nm) {
-                    const R c = (R)P.coef[a * MAXM + m - 1];
-                    d2 += c * ((R)(2 * m * m) * (R)(M_PI * M_PI)) * cm;
-                    acc[m] += (double)mr * (double)A * (double)m * M_PI * (double)sm;
-                    const R sn = sm * c1 + cm * s1, cn = cm * c1 - sm * s1;
-                    sm = sn;
-                    cm = cn;
-                }
-            }
-        }
-        V y;
-        y.x = lm.x - cddt * (g.x * invN + A * d2 * mr + hv);
-        y.y = lm.y - cddt * (g.y * invN);
-        lam_out[idx] = y;
-        acc[0] += (double)mr;
-    }
-    block_sum<NG, NTHR>(acc, red, tid);
-    if (tid == 0) {
-#pragma unroll
-        for (int q = 0; q < NG; ++q) gpart[q * gstride + blockIdx.x] = acc[q];
-    }
 }
 
 // Adds the per-block adjoint sums of one iteration into gacc[a][slot] in a
@@ -1286,261 +397,6 @@ __global__ void __launch_bounds__(NTHR)
     }
 }
 
-// ------------------------------------------------------------------ dispatch
-#define PFDD_SIZES(X) X(8) X(16) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(512)
-
-bool size_supported(int n)
-{
-#define PFDD_CASE(n_) if (n == n_) return true;
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return false;
-}
-
-bool systems_supported(int s) { return s == 1 || s == 2 || s == 3 || s == 4 || s == 6 || s == 12; }
-
-template <typename R>
-int tile_lines_rt(int n, int tq)
-{
-#define PFDD_CASE(n_)                                              \
-    if (n == n_) {                                                 \
-        if (tq == 0) return tile_lines_q<R, n_, 0>();              \
-        if (tq == 2) return tile_lines_q<R, n_, 2>();              \
-        return tile_lines_q<R, n_, 1>();                           \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return 0;
-}
-
-// Dispatch of a tile-shape choice tq (0, 1, 2) to the template argument.
-#define PFDD_TQ_DISPATCH(tq, ...)                                  \
-    switch (tq) {                                                  \
-    case 0: { constexpr int TQ = 0; __VA_ARGS__; } break;          \
-    case 2: { constexpr int TQ = 2; __VA_ARGS__; } break;          \
-    default: { constexpr int TQ = 1; __VA_ARGS__; } break;         \
-    }
-
-template <typename R, bool INV>
-hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo, long long nlines,
-                       const void *tw, const int *done, hipStream_t st, const uint8_t *zmask = nullptr,
-                       int tq = 1)
-{
-    typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                               \
-    if (n == n_) {                                                                                  \
-        PFDD_TQ_DISPATCH(tq, {                                                                      \
-            constexpr int NL = tile_lines_q<R, n_, TQ>();                                           \
-            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
-            PFDD_LAUNCH((k_fft_pass<R, n_, INV, TQ>), dim3(nb), dim3(NTHR), st,                     \
-                               (const V *)in, (V *)out, mi, mo, nlines, (const V *)tw, done, zmask); \
-        })                                                                                          \
-        return hipGetLastError();                                                                   \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn, const uint8_t *mask,
-                         const long long *prev, const void *amp, LineMap mi, LineMap mo, long long nlines,
-                         long long lps, const StepParams &P, const void *tw, double *partial,
-                         long long pstride, const int *done, hipStream_t st)
-{
-    typedef typename cvec<R>::type V;
-    // the coupled and Fourier core models get their own instantiation (one
-    // tile shape) so that the common sine-series kernel carries none of
-    // their registers
-    const bool gen = P.model == PFDD_CORE_EXTENDED || P.model == PFDD_CORE_PYRII || P.model == PFDD_CORE_FOURIER;
-#define PFDD_CASE(n_)                                                                                \
-    if (n == n_) {                                                                                   \
-        if (gen) {                                                                                   \
-            constexpr int NL = tile_lines_q<R, n_, 0>();                                             \
-            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
-            PFDD_LAUNCH((k_update<R, n_, 0, true>), dim3(nb), dim3(NTHR), st, (const V *)T1,         \
-                        (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, P,  \
-                        (const V *)tw, partial, pstride, done);                                      \
-            return hipGetLastError();                                                                \
-        }                                                                                            \
-        PFDD_TQ_DISPATCH(tq, {                                                                       \
-            constexpr int NL = tile_lines_q<R, n_, TQ>();                                            \
-            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
-            PFDD_LAUNCH((k_update<R, n_, TQ, false>), dim3(nb), dim3(NTHR), st, (const V *)T1,       \
-                               (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, \
-                               P, (const V *)tw, partial, pstride, done);                            \
-        })                                                                                           \
-        return hipGetLastError();                                                                    \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-hipError_t launch_adj_update(int n, int tq, const void *T1, const void *lam, const void *xs, void *lam_out,
-                             const uint8_t *mask, const void *amp, LineMap mi, LineMap mo, long long nlines,
-                             long long lps, const StepParams &P, const void *tw, double *gpart,
-                             long long gstride, const int *done, hipStream_t st)
-{
-    typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                                  \
-    if (n == n_) {                                                                                     \
-        PFDD_TQ_DISPATCH(tq, {                                                                         \
-            constexpr int NL = tile_lines_q<R, n_, TQ>();                                              \
-            const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                    \
-            hipLaunchKernelGGL((k_adj_update<R, n_, TQ>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,  \
-                               (const V *)lam, (const V *)xs, (V *)lam_out, mask, (const R *)amp, mi, mo, \
-                               nlines, lps, P, (const V *)tw, gpart, gstride, done);                   \
-        })                                                                                             \
-        return hipGetLastError();                                                                      \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return hipErrorInvalidValue;
-}
-
-template <typename R, int S>
-int mid_blocks_rt(int n, long long ncol)
-{
-#define PFDD_CASE(n_) if (n == n_) return (int)((ncol + mid_pairs<R, n_, S>() - 1) / mid_pairs<R, n_, S>());
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return 0;
-}
-
-template <typename R>
-int mid_blocks(int n, int S, long long ncol)
-{
-    switch (S) {
-    case 1: return mid_blocks_rt<R, 1>(n, ncol);
-    case 2: return mid_blocks_rt<R, 2>(n, ncol);
-    case 3: return mid_blocks_rt<R, 3>(n, ncol);
-    case 4: return mid_blocks_rt<R, 4>(n, ncol);
-    case 6: return mid_blocks_rt<R, 6>(n, ncol);
-    case 12: return mid_blocks_rt<R, 12>(n, ncol);
-    }
-    return 0;
-}
-
-template <typename R, int N, int S> constexpr bool mid_fits()
-{
-    return (long long)N * (S * mid_pairs<R, N, S>() + 1) * 2 * sizeof(R) <= 61440;
-}
-
-template <typename R, int S>
-hipError_t launch_mid_s(int n, int M, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
-                        double *part_el, const int *done, hipStream_t st)
-{
-    typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                             \
-    if (n == n_) {                                                                                \
-        if constexpr (mid_fits<R, n_, S>()) {                                                     \
-            constexpr int NP = mid_pairs<R, n_, S>();                                             \
-            const int bpm = (int)((ncol + NP - 1) / NP);                                          \
-            PFDD_LAUNCH((k_mid<R, R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,          \
-                               (V *)T2, (const R *)B, ncol, ntot, bpm, (const V *)tw, part_el, done); \
-            return hipGetLastError();                                                             \
-        } else {                                                                                  \
-            return hipErrorInvalidValue;                                                          \
-        }                                                                                         \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-bool mid_supported(int n, int S)
-{
-    auto chk = [&](auto sconst) -> bool {
-        constexpr int SS = decltype(sconst)::value;
-#define PFDD_CASE(n_) if (n == n_) return mid_fits<R, n_, SS>();
-        PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-        return false;
-    };
-    switch (S) {
-    case 1: return chk(std::integral_constant<int, 1>());
-    case 2: return chk(std::integral_constant<int, 2>());
-    case 3: return chk(std::integral_constant<int, 3>());
-    case 4: return chk(std::integral_constant<int, 4>());
-    case 6: return chk(std::integral_constant<int, 6>());
-    case 12: return chk(std::integral_constant<int, 12>());
-    }
-    return false;
-}
-
-template <typename R>
-hipError_t launch_mid(int n, int S, int M, void *T2, const void *B, long long ncol, long long ntot, const void *tw,
-                      double *part_el, const int *done, hipStream_t st)
-{
-    switch (S) {
-    case 1: return launch_mid_s<R, 1>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 2: return launch_mid_s<R, 2>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 3: return launch_mid_s<R, 3>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 4: return launch_mid_s<R, 4>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 6: return launch_mid_s<R, 6>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    case 12: return launch_mid_s<R, 12>(n, M, T2, B, ncol, ntot, tw, part_el, done, st);
-    }
-    return hipErrorInvalidValue;
-}
-
-template <typename R, int S>
-hipError_t launch_mid_otf_s(int n, int M, void *T2, const GParams &G, long long ncol, long long ntot,
-                            const void *tw, double *part_el, const int *done, hipStream_t st, int pipe, int ncu,
-                            int *nblocks)
-{
-    typedef typename cvec<R>::type V;
-#define PFDD_CASE(n_)                                                                             \
-    if (n == n_) {                                                                                \
-        if constexpr (mid_fits<R, n_, S>()) {                                                     \
-            constexpr int NP = mid_pairs<R, n_, S>();                                             \
-            const int bpm = (int)((ncol + NP - 1) / NP);                                          \
-            if (!pipe) {                                                                          \
-                *nblocks = M * bpm;                                                               \
-                PFDD_LAUNCH((k_mid_otf<R, n_, S>), dim3((unsigned)(M * bpm)), dim3(NTHR), st,      \
-                            (V *)T2, G, ncol, ntot, bpm, (const V *)tw, part_el, done);           \
-            } else {                                                                              \
-                /* as many blocks as fit on the device at once */                                 \
-                static const int occ = [] {                                                       \
-                    int o = 0;                                                                    \
-                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(                             \
-                            &o, (const void *)k_mid_otf_pipe<R, n_, S>, NTHR, 0) != hipSuccess)   \
-                        o = 1;                                                                    \
-                    return o > 0 ? o : 1;                                                         \
-                }();                                                                              \
-                const long long nt = (long long)M * bpm, cap = (long long)ncu * occ;             \
-                *nblocks = (int)(nt < cap ? nt : cap);                                            \
-                PFDD_LAUNCH((k_mid_otf_pipe<R, n_, S>), dim3((unsigned)*nblocks), dim3(NTHR), st,  \
-                            (V *)T2, G, ncol, ntot, bpm, (int)nt, (const V *)tw, part_el, done);  \
-            }                                                                                     \
-            return hipGetLastError();                                                             \
-        } else {                                                                                  \
-            return hipErrorInvalidValue;                                                          \
-        }                                                                                         \
-    }
-    PFDD_SIZES(PFDD_CASE)
-#undef PFDD_CASE
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-hipError_t launch_mid_otf(int n, int S, int M, void *T2, const GParams &G, long long ncol, long long ntot,
-                          const void *tw, double *part_el, const int *done, hipStream_t st, int pipe, int ncu,
-                          int *nblocks)
-{
-    switch (S) {
-    case 1: return launch_mid_otf_s<R, 1>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    case 2: return launch_mid_otf_s<R, 2>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    case 3: return launch_mid_otf_s<R, 3>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    case 4: return launch_mid_otf_s<R, 4>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    case 6: return launch_mid_otf_s<R, 6>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    case 12: return launch_mid_otf_s<R, 12>(n, M, T2, G, ncol, ntot, tw, part_el, done, st, pipe, ncu, nblocks);
-    }
-    return hipErrorInvalidValue;
-}
-
 } // namespace
 
 // ================================================================== handle
@@ -1568,6 +424,9 @@ struct pfdd_gl {
     int nb3 = 0, nb5 = 0, nb5max = 0; // blocks of pass 3 / pass 5 (current tile, smallest tile)
     int tq[4] = {1, 1, 1, 1};          // tile shape of fwd z, fwd y, inv y, inv z + update
     int xpipe = 0;                     // x pass: 0 one tile per block, 1 persistent pipelined
+    int plane = 0;                     // 1: three-kernel plane iteration (pfdd_plane.hip)
+    bool plane_ok = false;             // the grid and precision allow the plane iteration
+    int nbA = 0, nbC = 0, nb3pl = 0;   // grids of the plane iteration's kernels
     ThetaParams TP{};                  // core flag 5
     int ncu = 0;                       // compute units of the device
     double *stage2 = nullptr; // NSLOT * NB2
@@ -1694,6 +553,33 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
                            (const typename cvec<R>::type *)xo, (R *)h->cof, h->TP, h->N, h->done);
         HIPCHK(hipGetLastError());
     }
+    if (h->plane) {
+        // plane iteration: 2-D forward pass xo -> T1, x pass on T1 in its
+        // plane layout, 2-D inverse pass + update T1 + xo -> T2.  The slots
+        // of the line passes it does not have (1: fwd y, 3: inv y) get empty
+        // event pairs, so profiles keep their six-kernel shape.
+        const long long nplanes = (long long)h->SM * h->nx;
+        auto empty = [&](int k) -> int {
+            if (ev) {
+                HIPCHK(hipEventRecord(ev[2 * k], st));
+                HIPCHK(hipEventRecord(ev[2 * k + 1], st));
+            }
+            return PFDD_OK;
+        };
+        int rc;
+        arm(0);
+        HIPCHK(launch_plane_fwd<R>(h->ny, h->nz, xo, h->T1, nplanes, h->nbA, h->tw[1], h->tw[2], h->done, st));
+        t_ev_start = t_ev_stop = nullptr;
+        if ((rc = empty(1))) return rc;
+        arm(2);
+        HIPCHK(launch_mid_pl<R>(h->nx, h->S, h->M, h->T1, h->G, (long long)h->ny * h->nz, h->N, h->tw[0],
+                                h->partial + 4 * h->pstride, h->done, st, &h->nb3));
+        t_ev_start = t_ev_stop = nullptr;
+        if ((rc = empty(3))) return rc;
+        arm(4);
+        HIPCHK(launch_plane_upd<R>(h->ny, h->nz, h->T1, xo, T2, h->mask, h->prev, h->cof ? h->cof : h->amp, nplanes,
+                                   h->nx, h->nbC, P, h->tw[1], h->tw[2], h->partial, h->pstride, h->done, st));
+    } else {
     arm(0);
     HIPCHK((launch_pass<R, false>(h->nz, xo, h->T1, m.p1i, m.p1o, m.n1, h->tw[2], h->done, st, nullptr,
                                   h->tq[0])));
@@ -1714,6 +600,7 @@ int step_core(pfdd_gl *h, double tol, hipEvent_t *ev, const void *xo, void *T2, 
     HIPCHK(launch_update<R>(h->nz, h->tq[3], h->T1, xo, T2, h->mask, h->prev, h->cof ? h->cof : h->amp, m.p5i,
                             m.p5o, m.n5, (long long)h->nx * h->ny, P, h->tw[2], h->partial, h->pstride, h->done,
                             st));
+    }
     arm(5);
     // E_core as the reference accumulates it (sum / nx); core_energy_extended
     // and core_energy_pyrII do not accumulate it (fft_fftw_slab.cpp:1320,1483)
@@ -1743,11 +630,19 @@ void set_tiles(pfdd_gl *h, const int tq[5])
     // the smallest (fewest values per thread: no spills in FP32)
     const int m = h->P.model;
     if (m == PFDD_CORE_EXTENDED || m == PFDD_CORE_PYRII || m == PFDD_CORE_FOURIER) h->tq[3] = 0;
-    h->xpipe = h->stored_B ? 0 : tq[4];
-    const long long n5 = (long long)h->SM * h->nx * h->ny;
-    const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3])
-                                        : tile_lines_rt<float>(h->nz, h->tq[3]);
-    h->nb5 = (int)((n5 + nl - 1) / nl);
+    // tq[4]: 0 / 1 the five-kernel iteration with the plain / pipelined x
+    // pass, 2 the three-kernel plane iteration where the grid allows it
+    h->plane = (tq[4] == 2 && h->plane_ok) ? 1 : 0;
+    h->xpipe = (h->stored_B || tq[4] != 1) ? 0 : 1;
+    if (h->plane) {
+        h->nb5 = h->nbC;
+        h->nb3 = h->nb3pl;
+    } else {
+        const long long n5 = (long long)h->SM * h->nx * h->ny;
+        const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, h->tq[3])
+                                            : tile_lines_rt<float>(h->nz, h->tq[3]);
+        h->nb5 = (int)((n5 + nl - 1) / nl);
+    }
     ++h->version; // captured graphs hold the old launch shapes
 }
 
@@ -1816,13 +711,16 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
     int arg[5] = {1, 1, 1, 1, 0};
     int rc = PFDD_OK;
     // candidates: tile shape q = 0, 1, 2 with the one-tile-per-block x pass,
-    // then (on-the-fly contraction only) the pipelined x pass.  The clock of
-    // an idle GPU ramps up over the first few dozen dispatches, so all
+    // then (on-the-fly contraction only) the pipelined x pass, then (where
+    // the grid allows it) the three-kernel plane iteration.  The clock of an
+    // idle GPU ramps up over the first few dozen dispatches, so all
     // candidates are timed after a common warm-up, in two interleaved rounds,
-    // keeping each kernel's best time per candidate.
-    const int ncand = h->stored_B ? 3 : 4;
-    double tmin[4][5];
-    for (int c = 0; c < 4; ++c)
+    // keeping each kernel's best time per candidate.  The five-kernel
+    // iteration takes the best shape of each kernel; the plane iteration
+    // replaces it if its whole step (three kernels) is faster than that sum.
+    const int ncand = h->stored_B ? 3 : h->plane_ok ? 5 : 4;
+    double tmin[5][5];
+    for (int c = 0; c < 5; ++c)
         for (int p = 0; p < 5; ++p) tmin[c][p] = 1e300;
     {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
@@ -1831,7 +729,7 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
     }
     for (int round = 0; round < 2 && !rc; ++round)
         for (int c = 0; c < ncand && !rc; ++c) {
-            const int all[5] = {c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? 0 : 1};
+            const int all[5] = {c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? 0 : c - 2};
             set_tiles(h, all);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
             double warm[5], t[5];
@@ -1850,7 +748,15 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
                 }
             best[4] = tmin[q][4] < best[4] ? tmin[q][4] : best[4];
         }
-        if (ncand == 4 && tmin[3][4] < best[4]) arg[4] = 1;
+        if (ncand >= 4 && tmin[3][4] < best[4]) {
+            best[4] = tmin[3][4];
+            arg[4] = 1;
+        }
+        if (ncand == 5) {
+            const double five = best[0] + best[1] + best[2] + best[3] + best[4];
+            const double three = tmin[4][0] + tmin[4][3] + tmin[4][4];
+            if (three < five) arg[4] = 2;
+        }
     }
     HIPCHK(hipMemcpyAsync(h->xi[cur0], save, fb, hipMemcpyDeviceToDevice, h->stream));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
@@ -2390,6 +1296,25 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
         h->P.taud = h->taud;
     }
     h->pstride = h->nb5max > h->nb3 ? h->nb5max : h->nb3;
+    // the plane iteration: FP32 or FP64 grids with square y-z planes that fit
+    // in LDS and an x pass of the plane layout for nx; not with stored B
+    {
+        const long long ncol = (long long)h->ny * h->nz, nplanes = (long long)h->SM * h->nx;
+        const bool gen = d->core_model == PFDD_CORE_EXTENDED || d->core_model == PFDD_CORE_PYRII ||
+                         d->core_model == PFDD_CORE_FOURIER;
+        const bool f64 = h->prec == PFDD_FP64;
+        const bool sup = f64 ? plane_supported<double>(h->ny, h->nz) : plane_supported<float>(h->ny, h->nz);
+        h->nb3pl = f64 ? mid_pl_blocks<double>(h->nx, h->S, h->M, ncol) : mid_pl_blocks<float>(h->nx, h->S, h->M, ncol);
+        h->plane_ok = sup && h->nb3pl > 0 && !h->stored_B;
+        if (h->plane_ok) {
+            h->nbA = f64 ? plane_blocks<double>(h->ny, h->nz, nplanes, h->ncu, 0)
+                         : plane_blocks<float>(h->ny, h->nz, nplanes, h->ncu, 0);
+            h->nbC = f64 ? plane_blocks<double>(h->ny, h->nz, nplanes, h->ncu, gen ? 2 : 1)
+                         : plane_blocks<float>(h->ny, h->nz, nplanes, h->ncu, gen ? 2 : 1);
+            const int mx = h->nb3pl > h->nbC ? h->nb3pl : h->nbC;
+            if (mx > h->pstride) h->pstride = mx;
+        }
+    }
     {
         const int dflt[5] = {1, 1, 1, 1, 0};
         set_tiles(h, dflt); // nb5 of the model's update tile
@@ -2506,7 +1431,10 @@ int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5])
     if (!choice) return fail(PFDD_ERR_ARG, "null choice");
     for (int p = 0; p < 4; ++p)
         if (choice[p] < 0 || choice[p] > 2) return fail(PFDD_ERR_ARG, "tile shape choice must be 0, 1 or 2");
-    if (choice[4] < 0 || choice[4] > 1) return fail(PFDD_ERR_ARG, "x-pass choice must be 0 or 1");
+    if (choice[4] < 0 || choice[4] > 2) return fail(PFDD_ERR_ARG, "iteration choice must be 0, 1 or 2");
+    if (choice[4] == 2 && !h->plane_ok)
+        return fail(PFDD_ERR_UNSUPPORTED, "the plane iteration needs square y-z planes that fit in LDS "
+                                          "(FP32 up to 128 x 128, FP64 up to 80 x 80), nx of 16..128 and no stored B");
     const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, choice[3])
                                         : tile_lines_rt<float>(h->nz, choice[3]);
     if (h->tape.valid && ((long long)h->nx * h->ny) % nl != 0)
@@ -2704,7 +1632,6 @@ int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)
     return PFDD_OK;
 }
 
-
 int pfdd_gl_diag_fields(pfdd_gl *h, const double *sigma_rot, double ave_eps[9], double ave_sigma[9],
                         double *data_eps, double *data_sigma)
 {
@@ -2800,7 +1727,8 @@ int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step
     const double f = (double)h->SM * h->N * h->esz;
     const double bsz =
         h->stored_B ? (double)h->S * (h->S + 1) / 2 * h->N * (h->prec == PFDD_FP64 ? 8 : 4) : 0.0;
-    const double k[5] = {2 * f, 2 * f, 2 * f + bsz, 2 * f, 3 * f};
+    // the plane iteration has no separate y passes (slots 1 and 3)
+    const double k[5] = {2 * f, h->plane ? 0.0 : 2 * f, 2 * f + bsz, h->plane ? 0.0 : 2 * f, 3 * f};
     double tot = 0;
     for (int q = 0; q < 5; ++q) {
         if (bytes_per_kernel) bytes_per_kernel[q] = k[q];
