@@ -170,6 +170,18 @@ int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out);
  * stress-free strain average, stressfree_strain :3252-3326). */
 int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re);
 
+/* pfdd_gl_xi_sums without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under stream capture,
+ * tight loops, harnesses that forbid hipMalloc).  Same result as
+ * pfdd_gl_xi_sums, bit for bit.  Its block partials live in a fixed device
+ * array of (256 + 1) * PFDD_GL_NOALLOC_MAX_SYSTEMS doubles (2.06 MB, held on
+ * each GPU the library runs on until the process ends); handles with more
+ * than PFDD_GL_NOALLOC_MAX_SYSTEMS systems (nbatch * ns) get
+ * PFDD_ERR_UNSUPPORTED.  All calls on one GPU share that array: calls that
+ * overlap in time on the same GPU (other threads or streams) race. */
+#define PFDD_GL_NOALLOC_MAX_SYSTEMS 1024
+int pfdd_gl_xi_sums_noalloc(pfdd_gl *h, double *sum_re);
+
 /* Output-step averages: DiagStrain::compute + DiagStress::compute
  * (diag_strain.cpp:44-61, diag_stress.cpp:43-73) i.e. strain(),
  * total_average_strain(), stress() (fft_fftw_slab.cpp:3252-3553), for the

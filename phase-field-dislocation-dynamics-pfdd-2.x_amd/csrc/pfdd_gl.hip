@@ -287,6 +287,54 @@ __global__ void __launch_bounds__(NTHR)
     if (threadIdx.x == 0) part[(long long)a * gridDim.x + blockIdx.x] = v[0];
 }
 
+// Final stage of the per-system sums: system a = blockIdx.x adds its nb
+// block partials in index order with Kahan compensation, the same sequence of
+// operations as the host loop of pfdd_gl_xi_sums (additions only, so no
+// contraction into FMAs: the two give identical sums).
+__global__ void k_sys_sums_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const int a = blockIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double y = part[(long long)a * nb + b] - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    out[a] = s;
+}
+
+// Device scratch of pfdd_gl_xi_sums_noalloc: XS_NB block partials per system
+// for up to PFDD_GL_NOALLOC_MAX_SYSTEMS systems, then one sum per system.
+// Every slot that a call reads is written earlier in the same call, so the
+// array needs no clearing.  Host code never uses the symbol itself, only the
+// per-device address that noalloc_scratch() resolves.
+constexpr int XS_NB = 256;
+constexpr long long XS_SCRATCH = (long long)(XS_NB + 1) * PFDD_GL_NOALLOC_MAX_SYSTEMS;
+static __device__ __align__(256) double scratch[XS_SCRATCH];
+
+// Address of `scratch` on the current device, resolved once per device with
+// hipGetSymbolAddress (each GPU holds its own copy of the array).
+int noalloc_scratch(int device, double **ptr)
+{
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *ptr = it->second;
+        return PFDD_OK;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(PFDD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    addr[device] = (double *)p;
+    *ptr = (double *)p;
+    return PFDD_OK;
+}
+
 template <typename R>
 __global__ void k_from_f64(const double2 *__restrict__ src, typename cvec<R>::type *__restrict__ dst, long long n)
 {
@@ -1261,6 +1309,10 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     if (rc) return bail(rc);
     hipError_t e = hipSetDevice(d->device);
     if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "hipSetDevice(%d): %s", d->device, hipGetErrorString(e)));
+    {
+        double *sp = nullptr; // the device's no-allocation scratch, resolved at set-up
+        if ((rc = noalloc_scratch(d->device, &sp))) return bail(rc);
+    }
     if (d->stream) {
         h->stream = (hipStream_t)d->stream;
     } else {
@@ -1605,7 +1657,7 @@ int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!sum_re) return fail(PFDD_ERR_ARG, "null output");
-    const int nb = 256;
+    const int nb = XS_NB;
     double *part = nullptr;
     HIPCHK(hipMalloc(&part, sizeof(double) * nb * h->SM));
     if (h->prec == PFDD_FP64)
@@ -1629,6 +1681,31 @@ int pfdd_gl_xi_sums(pfdd_gl *h, double *sum_re)
         }
         sum_re[a] = s;
     }
+    return PFDD_OK;
+}
+
+int pfdd_gl_xi_sums_noalloc(pfdd_gl *h, double *sum_re)
+{
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!sum_re) return fail(PFDD_ERR_ARG, "null output");
+    if (h->SM > PFDD_GL_NOALLOC_MAX_SYSTEMS)
+        return fail(PFDD_ERR_UNSUPPORTED, "pfdd_gl_xi_sums_noalloc: %d systems exceed its scratch limit of %d",
+                    h->SM, PFDD_GL_NOALLOC_MAX_SYSTEMS);
+    double *part = nullptr;
+    if ((rc = noalloc_scratch(h->device, &part))) return rc;
+    double *sums = part + (long long)XS_NB * PFDD_GL_NOALLOC_MAX_SYSTEMS;
+    if (h->prec == PFDD_FP64)
+        hipLaunchKernelGGL((k_sys_sums<double>), dim3(XS_NB, h->SM), dim3(NTHR), 0, h->stream,
+                           (const double2 *)h->xi[h->cur], h->N, part);
+    else
+        hipLaunchKernelGGL((k_sys_sums<float>), dim3(XS_NB, h->SM), dim3(NTHR), 0, h->stream,
+                           (const float2 *)h->xi[h->cur], h->N, part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sys_sums_final, dim3(h->SM), dim3(64), 0, h->stream, (const double *)part, XS_NB, sums);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sum_re, sums, sizeof(double) * h->SM, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return PFDD_OK;
 }
 

@@ -22,6 +22,7 @@ CORE_NONE, CORE_PERFECT, CORE_SINE, CORE_MPEA, CORE_SERIES = 0, 1, 3, 7, 8
 CORE_EXTENDED, CORE_PYRII, CORE_BCC, CORE_FOURIER = 2, 4, 5, 9
 MAX_SYSTEMS = 12
 MAX_COEF = 8
+NOALLOC_MAX_SYSTEMS = 1024  # PFDD_GL_NOALLOC_MAX_SYSTEMS
 FLAG_STORED_B = 1
 FLAG_AUTOTUNE = 2
 STATUS = {0: "PFDD_OK", 1: "PFDD_ERR_ARG", 2: "PFDD_ERR_UNSUPPORTED", 3: "PFDD_ERR_HIP",
@@ -92,6 +93,7 @@ EXPORTS = [
     "pfdd_gl_set_xi", "pfdd_gl_get_xi", "pfdd_gl_xi_device", "pfdd_gl_set_xi_device",
     "pfdd_gl_get_xi_device",
     "pfdd_gl_set_tau", "pfdd_gl_set_core_coef", "pfdd_gl_step", "pfdd_gl_xi_sums",
+    "pfdd_gl_xi_sums_noalloc",
     "pfdd_gl_diag", "pfdd_gl_diag_fields", "pfdd_gl_profile", "pfdd_gl_traffic",
     "pfdd_gl_tape_record", "pfdd_gl_tape_backward", "pfdd_gl_tape_backward_device",
     "pfdd_gl_create_batch", "pfdd_gl_graph_create", "pfdd_gl_graph_run", "pfdd_gl_graph_destroy",
@@ -127,6 +129,7 @@ def load_library(path: str = LIB_PATH):
         "pfdd_gl_set_core_coef": (c_int, [vp, c_int, _dp]),
         "pfdd_gl_step": (c_int, [vp, c_int, c_double, ctypes.POINTER(GLReduce)]),
         "pfdd_gl_xi_sums": (c_int, [vp, _dp]),
+        "pfdd_gl_xi_sums_noalloc": (c_int, [vp, _dp]),
         "pfdd_gl_diag": (c_int, [vp, _dp, _dp, _dp]),
         "pfdd_gl_diag_fields": (c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "pfdd_gl_profile": (c_int, [vp, c_int, _dp, ip]),
@@ -305,6 +308,12 @@ class GLSolver:
     def xi_sums(self):
         out = np.zeros(self.SM)
         _check(self._lib.pfdd_gl_xi_sums(self._h, _ptr(out)))
+        return out
+
+    def xi_sums_noalloc(self):
+        """pfdd_gl_xi_sums_noalloc: the same sums with no device allocation."""
+        out = np.zeros(self.SM)
+        _check(self._lib.pfdd_gl_xi_sums_noalloc(self._h, _ptr(out)))
         return out
 
     def diag(self, sigma_rot):
