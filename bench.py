@@ -48,9 +48,12 @@ def solver_for(prob, precision):
     s = pfdd_gl.GLSolver(prob["shape"], prob["xn"], prob["xb"], prob["four"], prob["C44"],
                          prob["C12"], prob["C11"], prob["dslip"], prob["cddt"], S12=prob["S12"],
                          S44=prob["S44"], core_model=pfdd_gl.CORE_SINE, coef=prob["core_coef"],
-                         precision=precision, autotune=True, stored_B=STORED_B)
+                         precision=precision, stored_B=STORED_B)
     s.set_xi(prob["xi0"])
     s.set_tau(prob["tau"])
+    # tuning last (it restores the field): the GPU goes from the tuner's
+    # iterations straight into the warm-up without an idle gap
+    s.autotune()
     return s
 
 
@@ -74,12 +77,15 @@ def roofline(s, prof_steps=50, warm_steps=50):
     s.step(warm_steps)
     prof = s.profile(prof_steps)
     kbytes, step_bytes = s.traffic()
-    names = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"]
-    times = [prof[k] for k in names]
-    dom = int(np.argmax(times))
+    slots = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"]
+    plane = kbytes[1] == 0  # the three-kernel plane iteration has no separate y passes
+    names = ["fwd_yz", None, "x_contract", None, "inv_yz_update"] if plane else slots
+    live = [q for q in range(5) if names[q]]
+    times = {q: prof[slots[q]] for q in live}
+    dom = max(live, key=lambda q: times[q])
     achieved = kbytes[dom] / (times[dom] * 1e-3)
     per_kernel = {names[q]: {"ms": times[q], "bytes": kbytes[q],
-                             "GBps": kbytes[q] / (times[q] * 1e-3) / 1e9} for q in range(5)}
+                             "GBps": kbytes[q] / (times[q] * 1e-3) / 1e9} for q in live}
     return {
         "kernel": names[dom],
         "bound": "hbm",
@@ -91,6 +97,8 @@ def roofline(s, prof_steps=50, warm_steps=50):
         "per_kernel": per_kernel,
         "step_bytes": step_bytes,
         "step_ms_events": prof["step"],
+        "field_bytes": kbytes[0] / 2,
+        "iteration": "three-kernel plane" if plane else "five-kernel",
     }
 
 
@@ -135,7 +143,8 @@ def load_pmc_traffic(tag, kernel):
         return None
 
 
-KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update": "inv_z_update"}
+KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update": "inv_z_update",
+                "k_plane_fwd": "fwd_yz", "k_plane_upd": "inv_yz_update"}
 
 
 PROBE_STEPS = 3
@@ -236,7 +245,10 @@ def cpu_baseline(prob, nsteps=3):
     iterations of the same 128^3 x 12 workload timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    cores = min(16, os.cpu_count() or 1)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
     oracle.set_threads(cores)
     BB, _, _ = oracle.bmatrix(prob["shape"], prob["C44"], prob["C12"], prob["C11"], prob["dslip"],
                               prob["xn"], prob["xb"], prob["four"])
@@ -318,8 +330,6 @@ def bench_ensemble(args):
     for mbr, v0 in zip(members, twins):
         mbr["velocity_disorder_free_cells_per_step"] = v0
         mbr["mobility_ratio"] = mbr["velocity_cells_per_step"] / v0 if v0 > 0 else None
-        mbr["critical_stress_MPa_est"] = (mbr["stress_MPa"] * (1.0 - mbr["velocity_cells_per_step"] / v0)
-                                          if v0 > 0 else None)
     line = {
         "metric": "realization GL steps/s, batched alloy ensemble in one hipGraph",
         "value": nreal * nsteps / dt, "unit": "realization-steps/s", "n_gpus": 1, "steps": nsteps,
@@ -333,10 +343,99 @@ def bench_ensemble(args):
         "hbm": {"bytes_per_step": step_bytes, "achieved_GBps": step_bytes * nsteps / dt / 1e9,
                 "frac": step_bytes * nsteps / dt / PEAK_HBM},
         "steps_done": [r["steps_done"] for r in res],
-        "critical_stress_estimator": "linear mobility v = M (sigma - sigma_c) with M from the disorder-free "
-                                     "twin (same grid and stress, amplitude 1): sigma_c = sigma (1 - v / v_twin), "
-                                     "velocities over the second half of the run",
+        "critical_stress": "per realization from `bench.py --workload depinning` (stress bracketing)",
         "realizations": members,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_depinning(args, levels=8, rounds=3, smax_pa=700e6):
+    """Critical (depinning) stress of every realization of the alloy ensemble
+    by stress bracketing: each realization runs at `levels` shear stresses at
+    once (batch members sharing its amplitude field) for --steps iterations.
+    A level depins if the dislocation pair has swept a quarter of the box
+    (each line glided n/8 cells or more: the slipped area of system 0 on the
+    slip plane grew by n^2/4) or is still gliding in the last quarter of the
+    run (more than 0.05 cells per line); otherwise the pair is pinned.  The
+    critical stress lies between the highest level below which every level
+    stays pinned and the lowest level that depins.  The first round spaces the
+    levels over [0, smax], each later round over the bracket of the previous
+    one."""
+    base = setup.alloy_ensemble()
+    prec = pfdd_gl.PFDD_FP64 if args.precision == "fp64" else pfdd_gl.PFDD_FP32
+    m = setup.cu_like()
+    sl = setup.slip("basal3")
+    b0 = sl["xb"][0] / np.linalg.norm(sl["xb"][0])
+    n0 = sl["xn"][0] / np.linalg.norm(sl["xn"][0])
+    unit = np.outer(b0, n0) + np.outer(n0, b0)
+    brackets = [[(0.0, smax_pa)] * g["batch"] for g in base]
+    tsteps, tsec, nmem = 0, 0.0, 0
+    for rnd in range(rounds):
+        for gi, g in enumerate(base):
+            n, cnt, S = g["shape"][0], g["batch"], g["S"]
+            N = n ** 3
+            lv = []
+            for q in range(cnt):
+                lo, hi = brackets[gi][q]
+                lv.append(np.linspace(lo, hi, levels) if rnd == 0 else np.linspace(lo, hi, levels + 2)[1:-1])
+            amp_q = g["amp"].reshape(cnt, S * N)
+            amp = np.ascontiguousarray(np.repeat(amp_q, levels, axis=0)).reshape(-1)
+            xi0 = np.concatenate([setup.edge_dislocation_field(g["shape"], S)] * (cnt * levels))
+            tau = np.concatenate([setup.resolved_shear("basal3", (sp / m.mu) * unit, m.dslip)[0]
+                                  for q in range(cnt) for sp in lv[q]])
+            s = pfdd_gl.GLSolver(g["shape"], g["xn"], g["xb"], g["four"], g["C44"], g["C12"], g["C11"], g["dslip"],
+                                 g["cddt"], S12=g["S12"], S44=g["S44"], core_model=pfdd_gl.CORE_SERIES,
+                                 coef=g["coef"], ncoef=g["ncoef"], amp=amp, precision=prec,
+                                 batch=cnt * levels)
+            s.set_xi(xi0)
+            s.set_tau(tau)
+            a = s.xi_sums()
+            last = args.steps // 4
+            t0 = time.perf_counter()
+            s.step(args.steps - last)
+            tsec += time.perf_counter() - t0
+            c = s.xi_sums()
+            t0 = time.perf_counter()
+            s.step(last)
+            tsec += time.perf_counter() - t0
+            b = s.xi_sums()
+            s.close()
+            tsteps += args.steps * cnt * levels
+            nmem += cnt * levels
+            for q in range(cnt):
+                glide = [(b[(q * levels + k) * S] - a[(q * levels + k) * S]) / (2.0 * n) for k in range(levels)]
+                tail = [(b[(q * levels + k) * S] - c[(q * levels + k) * S]) / (2.0 * n) for k in range(levels)]
+                moved = [gl >= n / 8.0 or tl > 0.05 for gl, tl in zip(glide, tail)]
+                first = next((k for k in range(levels) if moved[k]), None)
+                lo_old, hi_old = brackets[gi][q]
+                if first is None:
+                    brackets[gi][q] = (float(lv[q][-1]), hi_old)
+                elif first == 0:
+                    brackets[gi][q] = (lo_old, float(lv[q][0]))
+                else:
+                    brackets[gi][q] = (float(lv[q][first - 1]), float(lv[q][first]))
+    out = []
+    for gi, g in enumerate(base):
+        for q in range(g["batch"]):
+            lo, hi = brackets[gi][q]
+            out.append({"grid": g["shape"][0], "critical_stress_MPa": 0.5 * (lo + hi) / 1e6,
+                        "bracket_MPa": [lo / 1e6, hi / 1e6]})
+    crit = np.array([r["critical_stress_MPa"] for r in out])
+    line = {
+        "metric": "critical (depinning) stress per alloy realization, stress bracketing in batched runs",
+        "value": tsteps / tsec, "unit": "realization-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic (lognormal GSF amplitude fields)",
+        "config": {"workload": "the 48 alloy realizations of the ensemble workload (16 x 48^3 + 24 x 64^3 + "
+                               "8 x 96^3, 3 basal systems, per-cell lognormal amplitude, straight edge pair), "
+                               f"{levels} shear levels per realization and round, {rounds} rounds, "
+                               f"{args.steps} iterations per level",
+                   "depinned_if": "each line glides >= n/8 cells within the run, or > 0.05 cells in its last quarter"},
+        "members_per_round": nmem // rounds,
+        "critical_stress_MPa": {"mean": float(crit.mean()), "std": float(crit.std()),
+                                "min": float(crit.min()), "max": float(crit.max())},
+        "realizations": out,
     }
     print(json.dumps(line))
     return 0
@@ -511,7 +610,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="workload for rocprofv3 --pmc runs (copy calibration + 3 steps)")
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -519,7 +618,7 @@ def main():
     ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles"))
     ap.add_argument("--streaming-ceiling", type=int, default=0, metavar="BYTES",
                     help="print torch's element-wise streaming rates over FP64 tensors of BYTES each (JSON)")
-    ap.add_argument("--workload", default="c1", choices=["c1", "ensemble", "adjoint", "frank-read"],
+    ap.add_argument("--workload", default="c1", choices=["c1", "ensemble", "depinning", "adjoint", "frank-read"],
                     help="c1: the 128^3 x 12 headline (default); the others print their own JSON line")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="ensemble precision")
     ap.add_argument("--stored-b", action="store_true",
@@ -541,6 +640,8 @@ def main():
         return 0
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "depinning":
+        return bench_depinning(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
     if args.workload == "frank-read":
@@ -582,7 +683,7 @@ def main():
     # A failed probe costs only this context field, never the bench line.
     ceil = {}
     for tag, o in out.items():
-        nb = int(o["roofline"]["per_kernel"]["fwd_z"]["bytes"]) // 2
+        nb = int(o["roofline"]["field_bytes"])
         try:
             child = subprocess.run([sys.executable, os.path.abspath(__file__), "--streaming-ceiling", str(nb)],
                                    capture_output=True, text=True, timeout=600)
@@ -596,7 +697,7 @@ def main():
         rate = {"line": ceil[tag]["mul (1 read + 1 write)"]["GBps"],
                 "update": ceil[tag]["add_ (2 reads + 1 write)"]["GBps"]}
         for role, k in o["roofline"]["per_kernel"].items():
-            k["frac_of_streaming_ceiling"] = k["GBps"] / rate["update" if role == "inv_z_update" else "line"]
+            k["frac_of_streaming_ceiling"] = k["GBps"] / rate["update" if "update" in role else "line"]
 
     f64, f32 = out["fp64"], out["fp32"]
     line = {
@@ -637,7 +738,8 @@ def main():
             tag: {"host_copy_ms": o["host_copy_ms"],
                   "steps_per_s": args.steps / (o["sec"] + 1e-3 * sum(o["host_copy_ms"].values()))}
             for tag, o in (("fp64", f64), ("fp32", f32))},
-        "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update", "x_contract (1 = pipelined)"],
+        "autotuned_tiles": {"passes": ["fwd_z", "fwd_y", "inv_y", "inv_z_update",
+                                       "iteration (0 five kernels, 1 with pipelined x pass, 2 three-kernel plane)"],
                             "fp64": f64["roofline"]["tiles"], "fp32": f32["roofline"]["tiles"]},
     }
     print(json.dumps(line))

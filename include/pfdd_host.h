@@ -82,6 +82,25 @@ int pfdd_host_cond_preset(const char *preset, double *par);
 int pfdd_host_cond_layers(const double *parA, const double *parB, int ny, double temperature, double *sigma,
                           double *dsigma);
 
+/* The same formulas on the caller's own arrays, for code that already holds
+ * the reference's geometry and slip tables (the fft_style hip plugin):
+ * rotate_stress (fft_fftw_slab.cpp:242-274) from prim / invprim (row-major)
+ * and norvol (fft.cpp:101-169); resolSS_Schmid (:737-769) and
+ * resolSS_non_Schmid (:774-845) from the rotated slip vectors xn, xb (ns*3);
+ * core_energy_USFE_angle_tau's An' (:1500-1545) with m->mu_s, dslip, b and the
+ * fit par5 = a_slope, a_b, b_b, c_slope, c_b; project_core_energy
+ * (:1550-1624) of plane 0 from xi (fft->xi[0] layout) into delta, ddelta (N). */
+int pfdd_host_rotate_stress_raw(const double *prim9, const double *invprim9, double norvol, const double *sigma9,
+                                double *sigma_rot9);
+int pfdd_host_resolss_raw(int ns, const double *xn, const double *xb, const double *sigma_rot9, double dslip,
+                          double *tau);
+int pfdd_host_resolss_non_schmid_raw(int ns, const double *xb, const double *sigma_rot9, double dslip, double angle,
+                                     double w1, double w2, double w3, double *tau);
+double pfdd_host_usfe_angle_tau_raw(const pfdd_host_material *m, const double *par5, const double *sigma_rot9,
+                                    double angle);
+int pfdd_host_project_core_energy(int nx, int ny, int nz, int ns, const double *xb, const double *xi, double *delta,
+                                  double *ddelta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -505,6 +505,27 @@ __device__ __forceinline__ void tile_store(V *__restrict__ dst, const Tile<V> ld
 }
 
 // ------------------------------------------------------------- reductions
+// Compensated (Kahan) running sum of doubles, for sums that run over many
+// terms in one thread (k-points of the x pass, planes of the plane update).
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double y = x - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// Error-free sum a + b = s + e (Knuth's TwoSum, no ordering condition).
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
 // Block-wide sum of NV doubles (NTHR threads, wave64).  Result valid in
 // thread 0.  Uses `red` (NV * NTHR/64 doubles of LDS).
 template <int NV, int NTHR>
@@ -531,6 +552,46 @@ __device__ __forceinline__ void block_sum(double *v, double *red, int tid)
             double s = 0.0;
             for (int i = 0; i < NW; ++i) s += red[q * NW + i];
             v[q] = s;
+        }
+    }
+}
+
+// Block-wide sum of NV doubles (NTHR threads, wave64) for the energy and
+// average reductions: a pairwise shuffle tree within each wave (6 levels,
+// relative error at most 6 u for terms of one sign), then the wave sums
+// added with TwoSum and their rounding errors added back.  Compensating every
+// shuffle level as well costs as much as 8 % of the update kernel (measured
+// at 128^3 x 12) for no visible gain.  Result valid in thread 0.  Uses `red`
+// (NV * NTHR/64 doubles of LDS).
+template <int NV, int NTHR>
+__device__ __forceinline__ void block_sum_comp(double *v, double *red, int tid)
+{
+    constexpr int NW = NTHR / 64;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[q] = x;
+    }
+    const int lane = tid & 63, w = tid >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) red[q * NW + w] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0, e = 0.0;
+            for (int i = 0; i < NW; ++i) {
+                double t, r;
+                two_sum(s, red[q * NW + i], t, r);
+                s = t;
+                e += r;
+            }
+            v[q] = s + e;
         }
     }
 }

@@ -473,6 +473,16 @@ struct pfdd_gl {
     int tq[4] = {1, 1, 1, 1};          // tile shape of fwd z, fwd y, inv y, inv z + update
     int xpipe = 0;                     // x pass: 0 one tile per block, 1 persistent pipelined
     int plane = 0;                     // 1: three-kernel plane iteration (pfdd_plane.hip)
+    // pfdd_gl_step's own graph: SG_ITERS iterations captured on the handle's
+    // stream, replayed for long runs; captured again when the kernel
+    // arguments (version), the current ping-pong buffer or tol change
+    struct SelfGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned ver = 0;
+        int cur = -1;
+        double tol = 0.0;
+    } sg;
     bool plane_ok = false;             // the grid and precision allow the plane iteration
     int nbA = 0, nbC = 0, nb3pl = 0;   // grids of the plane iteration's kernels
     ThetaParams TP{};                  // core flag 5
@@ -683,7 +693,7 @@ void set_tiles(pfdd_gl *h, const int tq[5])
     h->plane = (tq[4] == 2 && h->plane_ok) ? 1 : 0;
     h->xpipe = (h->stored_B || tq[4] != 1) ? 0 : 1;
     if (h->plane) {
-        h->nb5 = h->nbC;
+        h->nb5 = (int)((long long)h->SM * h->nx); // one partial per y-z plane
         h->nb3 = h->nb3pl;
     } else {
         const long long n5 = (long long)h->SM * h->nx * h->ny;
@@ -868,11 +878,20 @@ int set_device(const pfdd_gl *h)
     return PFDD_OK;
 }
 
+void self_graph_release(pfdd_gl *h)
+{
+    if (h->sg.exec) (void)hipGraphExecDestroy(h->sg.exec);
+    if (h->sg.graph) (void)hipGraphDestroy(h->sg.graph);
+    h->sg.exec = nullptr;
+    h->sg.graph = nullptr;
+}
+
 void destroy_all(pfdd_gl *h)
 {
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    self_graph_release(h);
     void *ptrs[] = {h->cof, h->taud, h->xi[0], h->xi[1], h->T1, h->B, h->tw[0], h->tw[1], h->tw[2], h->mask, h->prev,
                     h->amp, h->partial, h->res, h->done, h->steps, h->stage2, h->ticket, h->dw[0], h->dw[1],
                     h->deps, h->dsig, h->tape.lam[0], h->tape.lam[1], h->tape.gpart, h->tape.gacc};
@@ -1233,6 +1252,39 @@ int tape_backward_any(pfdd_gl *h, const void *dxi_T, bool dev_in, void *dxi_0, b
     return PFDD_OK;
 }
 
+// Iterations per replay of pfdd_gl_step's graph (even: the ping-pong buffer
+// is the same before and after a replay).
+constexpr int SG_ITERS = 10;
+
+// Captures (if stale) SG_ITERS GL iterations of h on its own stream into
+// h->sg.  Capturing enqueues nothing; h->cur is back where it started.
+int self_graph_ready(pfdd_gl *h, double tol)
+{
+    if (h->sg.exec && h->sg.ver == h->version && h->sg.cur == h->cur && h->sg.tol == tol) return PFDD_OK;
+    self_graph_release(h);
+    const int cur0 = h->cur;
+    const bool stepped0 = h->stepped;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PFDD_OK;
+    for (int it = 0; it < SG_ITERS && !rc; ++it)
+        rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    h->cur = cur0;
+    h->stepped = stepped0;
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(PFDD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    h->sg.graph = graph;
+    HIPCHK(hipGraphInstantiate(&h->sg.exec, graph, nullptr, nullptr, 0));
+    h->sg.ver = h->version;
+    h->sg.cur = h->cur;
+    h->sg.tol = tol;
+    return PFDD_OK;
+}
+
 } // namespace
 
 // ===================================================================== C ABI
@@ -1363,7 +1415,7 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
                          : plane_blocks<float>(h->ny, h->nz, nplanes, h->ncu, 0);
             h->nbC = f64 ? plane_blocks<double>(h->ny, h->nz, nplanes, h->ncu, gen ? 2 : 1)
                          : plane_blocks<float>(h->ny, h->nz, nplanes, h->ncu, gen ? 2 : 1);
-            const int mx = h->nb3pl > h->nbC ? h->nb3pl : h->nbC;
+            const long long mx = h->nb3pl > nplanes ? h->nb3pl : nplanes;
             if (mx > h->pstride) h->pstride = mx;
         }
     }
@@ -1436,6 +1488,7 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "setup: %s", hipGetErrorString(e)));
     if (d->flags & PFDD_GL_FLAG_AUTOTUNE) {
         rc = h->prec == PFDD_FP64 ? autotune<double>(h, 4, nullptr) : autotune<float>(h, 4, nullptr);
+        if (!rc) rc = self_graph_ready(h, 0.0);
         if (rc) return bail(rc);
     }
     *out = h;
@@ -1500,7 +1553,10 @@ int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5])
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1) return fail(PFDD_ERR_ARG, "nsteps must be >= 1");
-    return h->prec == PFDD_FP64 ? autotune<double>(h, nsteps, choice) : autotune<float>(h, nsteps, choice);
+    rc = h->prec == PFDD_FP64 ? autotune<double>(h, nsteps, choice) : autotune<float>(h, nsteps, choice);
+    // the tuned launch shapes are final: capture pfdd_gl_step's graph now
+    // (tol 0) rather than inside the caller's first long run
+    return rc ? rc : self_graph_ready(h, 0.0);
 }
 
 int pfdd_gl_destroy(pfdd_gl *h)
@@ -1590,7 +1646,18 @@ int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out)
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
     HIPCHK(hipGetLastError());
     const int cur0 = h->cur;
-    for (int s = 0; s < nsteps; ++s) {
+    int eager = nsteps;
+    // long runs replay a graph of SG_ITERS iterations (one launch instead of
+    // 6 SG_ITERS), unless the caller is capturing this stream itself
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(h->stream, &cs));
+    if (nsteps >= 2 * SG_ITERS && cs == hipStreamCaptureStatusNone) {
+        if ((rc = self_graph_ready(h, tol))) return rc;
+        for (int r = 0; r < nsteps / SG_ITERS; ++r) HIPCHK(hipGraphLaunch(h->sg.exec, h->stream));
+        h->stepped = true;
+        eager = nsteps % SG_ITERS;
+    }
+    for (int s = 0; s < eager; ++s) {
         rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
         if (rc) return rc;
     }
@@ -1827,6 +1894,8 @@ int pfdd_gl_traffic(pfdd_gl *h, double *bytes_per_kernel, double *bytes_per_step
 struct pfdd_gl_graph {
     std::vector<pfdd_gl *> hs;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> branch; // one capture stream per handle: parallel branches of the graph
+    std::vector<hipEvent_t> join;    // fork / join events of the capture (one per handle, plus the fork)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<unsigned> ver;
@@ -1845,24 +1914,34 @@ void graph_release(pfdd_gl_graph *g)
     g->graph = nullptr;
 }
 
+// The graph forks at its root into one branch per handle (the handle's
+// iterations, in order, on its own capture stream) and joins at the end, so
+// the kernels of different handles -- e.g. the small grids of an ensemble --
+// may run concurrently.
 int graph_capture(pfdd_gl_graph *g, double tol)
 {
     graph_release(g);
-    std::vector<hipStream_t> saved(g->hs.size());
-    for (size_t i = 0; i < g->hs.size(); ++i) {
+    const size_t n = g->hs.size();
+    std::vector<hipStream_t> saved(n);
+    for (size_t i = 0; i < n; ++i) {
         saved[i] = g->hs[i]->stream;
-        g->hs[i]->stream = g->stream;
+        g->hs[i]->stream = g->branch[i];
     }
     int rc = PFDD_OK;
     hipError_t e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
         rc = fail(PFDD_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     } else {
-        for (int it = 0; it < g->iters && !rc; ++it)
-            for (pfdd_gl *h : g->hs) {
+        e = hipEventRecord(g->join[n], g->stream); // fork
+        for (size_t i = 0; i < n && !rc && e == hipSuccess; ++i) {
+            pfdd_gl *h = g->hs[i];
+            e = hipStreamWaitEvent(h->stream, g->join[n], 0);
+            for (int it = 0; it < g->iters && !rc && e == hipSuccess; ++it)
                 rc = h->prec == PFDD_FP64 ? step_once<double>(h, tol, nullptr) : step_once<float>(h, tol, nullptr);
-                if (rc) break;
-            }
+            if (!rc && e == hipSuccess) e = hipEventRecord(g->join[i], h->stream);
+            if (!rc && e == hipSuccess) e = hipStreamWaitEvent(g->stream, g->join[i], 0);
+        }
+        if (!rc && e != hipSuccess) rc = fail(PFDD_ERR_HIP, "graph fork/join: %s", hipGetErrorString(e));
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(g->stream, &graph);
         if (!rc && e != hipSuccess) rc = fail(PFDD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -1901,11 +1980,15 @@ int pfdd_gl_graph_create(pfdd_gl *const *handles, int n, int iters, pfdd_gl_grap
     g->iters = iters;
     g->ver.assign(n, 0);
     g->cur.assign(n, -1);
+    g->branch.assign(n, nullptr);
+    g->join.assign(n + 1, nullptr);
     hipError_t e = hipSetDevice(handles[0]->device);
     if (e == hipSuccess) e = hipStreamCreate(&g->stream);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipStreamCreate(&g->branch[i]);
+    for (int i = 0; i <= n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete g;
-        return fail(PFDD_ERR_HIP, "graph stream: %s", hipGetErrorString(e));
+        pfdd_gl_graph_destroy(g);
+        return fail(PFDD_ERR_HIP, "graph streams: %s", hipGetErrorString(e));
     }
     *out = g;
     return PFDD_OK;
@@ -1958,6 +2041,10 @@ int pfdd_gl_graph_destroy(pfdd_gl_graph *g)
     hipSetDevice(g->hs[0]->device);
     if (g->stream) hipStreamSynchronize(g->stream);
     graph_release(g);
+    for (hipStream_t b : g->branch)
+        if (b) (void)hipStreamDestroy(b);
+    for (hipEvent_t ev : g->join)
+        if (ev) (void)hipEventDestroy(ev);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
     return PFDD_OK;

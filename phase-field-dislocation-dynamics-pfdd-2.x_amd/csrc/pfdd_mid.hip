@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(NTHR)
     }
     __syncthreads();
     fft_stages<N, 1, NL, NTHR, false>(lds, tw, tid);
-    double el = 0.0;
+    KSum el; // compensated
     for (int e = tid; e < NP * N; e += NTHR) {
         const int q = e / N, p = e % N;
         if (c0 + q >= ncol) continue;
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(NTHR)
         (void)NPAIR;
 #pragma unroll
         for (int a = 0; a < S; ++a) {
-            el += (double)x[a].x * (double)g[a][0] + (double)x[a].y * (double)g[a][1];
+            el.add((double)x[a].x * (double)g[a][0] + (double)x[a].y * (double)g[a][1]);
             lds.st(p * LDP + a * NP + q, mk<V>(g[a][0], g[a][1]));
         }
     }
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(NTHR)
         const int a = l / NP, q = l % NP;
         if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
     }
-    double v[1] = {el};
+    double v[1] = {el.s};
     block_sum<1, NTHR>(v, red, tid);
     if (tid == 0) part_el[blockIdx.x] = v[0];
 }
@@ -272,7 +272,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
     fft_stages_x<N, 1, NL, NTHR, false>(lds, tw, tid);
     // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
     // replaces xihat_a in the tile
-    double el = 0.0;
+    KSum el; // compensated
     if constexpr (TPP == 1) {
         // one thread per k-point: it reads all xihat_b of the point, then
         // overwrites them with ghat_a
@@ -285,7 +285,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
             otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, [&](int a, T gr, T gi) {
                 const int idx = p * LDP + a * NP + q;
                 const V x = lds.ld(idx);
-                el += (double)x.x * gr + (double)x.y * gi;
+                el.add((double)x.x * gr + (double)x.y * gi);
                 lds.st(idx, mk<V>((R)gr, (R)gi));
             });
         }
@@ -315,14 +315,14 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
             for (int aa = 0; aa < SA; ++aa) {
                 const int idx = p * LDP + (h * SA + aa) * NP + q;
                 const V x = lds.ld(idx);
-                el += (double)x.x * g[aa][0] + (double)x.y * g[aa][1];
+                el.add((double)x.x * g[aa][0] + (double)x.y * g[aa][1]);
                 lds.st(idx, mk<V>((R)g[aa][0], (R)g[aa][1]));
             }
         }
     }
     __syncthreads();
     fft_stages_x<N, 1, NL, NTHR, true>(lds, tw, tid);
-    return el;
+    return el.s;
 }
 
 template <typename R, int S>

@@ -115,8 +115,8 @@ __global__ void __launch_bounds__((plane_threads<R, NY, NZ>()))
 }
 
 // C: inverse 2-D FFT of every plane of T fused with the GL update of the
-// plane's cells (gl_update_cell), per-block sums of xinorm^2, xiave, the core
-// energy and the stress work.  A plane holds one system, so tau and the core
+// plane's cells (gl_update_cell), and per-plane sums of xinorm^2, xiave, the
+// core energy and the stress work (partial[slot * pstride + plane]).  A plane holds one system, so tau and the core
 // coefficients are uniform over it.
 template <typename R, int NY, int NZ, bool GEN>
 __global__ void __launch_bounds__((plane_threads<R, NY, NZ>()))
@@ -141,7 +141,6 @@ __global__ void __launch_bounds__((plane_threads<R, NY, NZ>()))
     plane_twiddles<NY, NZ, NT>(tws, twy, twz, tid);
     const R cddt = (R)P.cddt, invN = (R)P.invN;
     const int nm = P.ncoef;
-    double s_norm = 0.0, s_ave = 0.0, s_core = 0.0, s_str = 0.0;
     // The next plane of T is in flight in registers during the transform and
     // the update of the current one; the old field is read by the update loop
     // itself, a few cells per thread at a time (holding it in registers
@@ -163,6 +162,9 @@ __global__ void __launch_bounds__((plane_threads<R, NY, NZ>()))
         fft_stages_g<NZ, 1, NY, 1, LDY, NT, true>(lds, tws + NY, tid);
         const long long base = pl * TOT;
         const R tau = (R)(P.taud ? P.taud[a] : P.tau[a]);
+        // plain FP64 sums over the thread's cells of the plane (at most 16),
+        // the block tree of block_sum_comp, one partial per plane
+        double p_norm = 0.0, p_ave = 0.0, p_core = 0.0, p_str = 0.0;
 #pragma unroll UNR
         for (int e = tid; e < TOT; e += NT) {
             const long long idx = base + e;
@@ -173,20 +175,20 @@ __global__ void __launch_bounds__((plane_threads<R, NY, NZ>()))
             const V y = gl_update_cell<R, GEN>(g, x, idx, tau, as % 3, [&](int m) { return lcf[m]; }, P, nm,
                                                cddt, invN, mask, prev, xo, amp, dn, dens);
             st_stream(&xn[idx], y);
-            s_norm += dn;
-            s_ave += (double)y.x;
-            s_core += (double)dens;
-            s_str += (double)tau * (double)x.x;
+            p_norm += dn;
+            p_ave += (double)y.x;
+            p_core += (double)dens;
+            p_str += (double)tau * (double)x.x;
+        }
+        double v[4] = {p_norm, p_ave, p_core, p_str};
+        block_sum_comp<4, NT>(v, red, tid);
+        if (tid == 0) {
+            partial[0 * pstride + pl] = v[0];
+            partial[1 * pstride + pl] = v[1];
+            partial[2 * pstride + pl] = v[2];
+            partial[3 * pstride + pl] = v[3];
         }
         __syncthreads();
-    }
-    double v[4] = {s_norm, s_ave, s_core, s_str};
-    block_sum<4, NT>(v, red, tid);
-    if (tid == 0) {
-        partial[0 * pstride + blockIdx.x] = v[0];
-        partial[1 * pstride + blockIdx.x] = v[1];
-        partial[2 * pstride + blockIdx.x] = v[2];
-        partial[3 * pstride + blockIdx.x] = v[3];
     }
 }
 

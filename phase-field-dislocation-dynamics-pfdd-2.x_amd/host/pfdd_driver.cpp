@@ -54,25 +54,7 @@ void FFT_HIP::project_core_energy()
     const long long N = (long long)nx * ny * nz;
     delta.assign(N, 0.0);
     ddelta.assign(N, 0.0);
-    const std::vector<double> &b = slip.xb;
-    for (long long r = 0; r < N; ++r) {
-        if (slip_systems == 1 || slip_systems == 2) {
-            delta[r] = xi[2 * r];
-        } else {
-            double v = 0.0;
-            for (int c = 0; c < 3; ++c) {
-                double p = 0.0;
-                for (int s = 0; s < 3; ++s) p += xi[2 * (r + s * N)] * b[3 * s + c];
-                v += p * b[3 * 1 + c];
-            }
-            delta[r] = v;
-        }
-    }
-    const int j = ny / 2, k = nz / 2;
-    for (int i = 0; i < nx; ++i) {
-        const long long q = (long long)i * ny * nz + (long long)j * nz + k;
-        ddelta[q] = i + 1 == nx ? 0.0 : (delta[q + (long long)ny * nz] - delta[q]) / 1.0;
-    }
+    pfdd_host::project_core_energy(nx, ny, nz, slip_systems, slip.xb.data(), xi.data(), delta.data(), ddelta.data());
 }
 
 void FFT_HIP::theta(std::vector<double> &th)

@@ -460,6 +460,31 @@ double usfe_angle_tau(const Material &m, const double sigma_rot[3][3], double an
     return g / (m.mu_s * m.dslip * m.b);
 }
 
+void project_core_energy(int nx, int ny, int nz, int ns, const double *xb, const double *xi, double *delta,
+                         double *ddelta)
+{
+    const long long N = (long long)nx * ny * nz;
+    for (long long r = 0; r < N; ++r) delta[r] = ddelta[r] = 0.0;
+    for (long long r = 0; r < N; ++r) {
+        if (ns == 1 || ns == 2) {
+            delta[r] = xi[2 * r];
+        } else {
+            double v = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double p = 0.0;
+                for (int s = 0; s < 3; ++s) p += xi[2 * (r + s * N)] * xb[3 * s + c];
+                v += p * xb[3 * 1 + c];
+            }
+            delta[r] = v;
+        }
+    }
+    const int j = ny / 2, k = nz / 2;
+    for (int i = 0; i < nx; ++i) {
+        const long long q = (long long)i * ny * nz + (long long)j * nz + k;
+        ddelta[q] = i + 1 == nx ? 0.0 : (delta[q + (long long)ny * nz] - delta[q]) / 1.0;
+    }
+}
+
 std::vector<double> core_coefficients(const Material &m, int flag)
 {
     const double mdb = m.mu_s * m.dslip * m.b;

@@ -86,6 +86,15 @@ void resolss_non_schmid(const SlipTable &t, const double sigma_rot[3][3], double
 // An' of core_energy_USFE_angle_tau (fft_fftw_slab.cpp:1500-1545).
 double usfe_angle_tau(const Material &m, const double sigma_rot[3][3], double angle);
 
+// project_core_energy (fft_fftw_slab.cpp:1550-1624) for plane 0, the one the
+// apps' compute_delta / compute_ddelta read: delta = Re xi_0 for one or two
+// systems, else the projection of the plane's three systems onto b_1;
+// ddelta = forward difference along x on the line j = ny/2, k = nz/2 (dx = 1),
+// zero at the last cell and off the line.  delta, ddelta: N values, zeroed
+// here first; xi in the fft->xi[0] layout; xb: ns*3.
+void project_core_energy(int nx, int ny, int nz, int ns, const double *xb, const double *xi, double *delta,
+                         double *ddelta);
+
 // Core-energy coefficients normalised as core_energy_* do (fft_fftw_slab.cpp:
 // 1165-1178, 1396-1406): flag 1 -> {An'}, flag 3 -> {isf', Cn'} with Cn' =
 // (usf - isf'/2)/(mu dslip b), the reference's own expression.

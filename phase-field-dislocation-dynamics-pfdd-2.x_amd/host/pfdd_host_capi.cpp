@@ -207,3 +207,79 @@ extern "C" int pfdd_host_cond_layers(const double *parA, const double *parB, int
     for (int j = 0; j < ny; ++j) sigma[j] = s[j], dsigma[j] = d[j];
     return 0;
 }
+
+// ------------------------------------------- raw-array forms for the plugin
+extern "C" int pfdd_host_rotate_stress_raw(const double *prim9, const double *invprim9, double norvol,
+                                           const double *sigma9, double *sigma_rot9)
+{
+    if (!prim9 || !invprim9 || !sigma9 || !sigma_rot9) return 1;
+    Geometry g;
+    double s[3][3], sr[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            g.prim[i][j] = prim9[3 * i + j];
+            g.invprim[i][j] = invprim9[3 * i + j];
+            s[i][j] = sigma9[3 * i + j];
+        }
+    g.norvol = norvol;
+    rotate_stress(g, s, sr);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sigma_rot9[3 * i + j] = sr[i][j];
+    return 0;
+}
+
+static void table_from(int ns, const double *xn, const double *xb, SlipTable &t)
+{
+    t.slip_systems = ns;
+    t.xn.assign(xn ? xn : xb, (xn ? xn : xb) + 3 * ns);
+    t.xb.assign(xb, xb + 3 * ns);
+}
+
+extern "C" int pfdd_host_resolss_raw(int ns, const double *xn, const double *xb, const double *sigma_rot9,
+                                     double dslip, double *tau)
+{
+    if (ns < 1 || !xn || !xb || !sigma_rot9 || !tau || dslip == 0.0) return 1;
+    SlipTable t;
+    table_from(ns, xn, xb, t);
+    double sr[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sr[i][j] = sigma_rot9[3 * i + j];
+    resolss(t, sr, dslip, tau);
+    return 0;
+}
+
+extern "C" int pfdd_host_resolss_non_schmid_raw(int ns, const double *xb, const double *sigma_rot9, double dslip,
+                                                double angle, double w1, double w2, double w3, double *tau)
+{
+    if (ns < 1 || !xb || !sigma_rot9 || !tau || dslip == 0.0) return 1;
+    SlipTable t;
+    table_from(ns, nullptr, xb, t);
+    double sr[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sr[i][j] = sigma_rot9[3 * i + j];
+    resolss_non_schmid(t, sr, dslip, angle, w1, w2, w3, tau);
+    return 0;
+}
+
+extern "C" double pfdd_host_usfe_angle_tau_raw(const pfdd_host_material *m, const double *par5,
+                                               const double *sigma_rot9, double angle)
+{
+    if (!m || !par5 || !sigma_rot9) return 0.0;
+    Material mat;
+    mat.mu_s = m->mu_s;
+    mat.dslip = m->dslip;
+    mat.b = m->b;
+    mat.a_slope = par5[0], mat.a_b = par5[1], mat.b_b = par5[2], mat.c_slope = par5[3], mat.c_b = par5[4];
+    double sr[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sr[i][j] = sigma_rot9[3 * i + j];
+    return usfe_angle_tau(mat, sr, angle);
+}
+
+extern "C" int pfdd_host_project_core_energy(int nx, int ny, int nz, int ns, const double *xb, const double *xi,
+                                             double *delta, double *ddelta)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || ns < 1 || !xb || !xi || !delta || !ddelta) return 1;
+    project_core_energy(nx, ny, nz, ns, xb, xi, delta, ddelta);
+    return 0;
+}

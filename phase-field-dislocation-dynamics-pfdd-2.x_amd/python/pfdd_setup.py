@@ -45,6 +45,10 @@ def lib():
             "pfdd_host_init_sxtal": [cp, ci, ci, ci, ci, _dp, _dp],
             "pfdd_host_cond_preset": [cp, _dp],
             "pfdd_host_cond_layers": [_dp, _dp, ci, cd, _dp, _dp],
+            "pfdd_host_rotate_stress_raw": [_dp, _dp, cd, _dp, _dp],
+            "pfdd_host_resolss_raw": [ci, _dp, _dp, _dp, cd, _dp],
+            "pfdd_host_resolss_non_schmid_raw": [ci, _dp, _dp, cd, cd, cd, cd, cd, _dp],
+            "pfdd_host_project_core_energy": [ci, ci, ci, ci, _dp, _dp, _dp, _dp],
         }.items():
             f = getattr(L, name)
             f.restype = ci

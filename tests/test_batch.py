@@ -163,3 +163,36 @@ def test_every_tuning_choice_gives_the_same_field(precision, shape, systems):
         s.step(12)
         assert np.array_equal(s.get_xi(), x_ref), choice
         s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_step_graph_replay_equals_single_steps(precision):
+    print(f"case: pfdd_gl_step graph replay (45 and 20 iterations) vs single iterations, precision {precision}",
+          flush=True)
+    c = P.make_case(shape=(32, 32, 32), noise=1e-2, sigma_mu=0.004)
+    a = P.gpu_solver(c, precision=precision)
+    b = P.gpu_solver(c, precision=precision)
+    try:
+        ra = a.step(45)  # four replays of a 10-iteration graph, then 5 eager iterations
+        for _ in range(45):
+            rb = b.step(1)
+        assert ra["steps_done"] == 45
+        assert np.array_equal(a.get_xi(), b.get_xi())
+        assert ra["e_total"] == rb["e_total"] and ra["xinorm"] == rb["xinorm"]
+        # new kernel arguments (tau) and a new tuning: the graph is captured again
+        tau2 = 1.5 * np.asarray(c["tau"])
+        a.set_tau(tau2)
+        b.set_tau(tau2)
+        a.step(20)
+        for _ in range(20):
+            b.step(1)
+        assert np.array_equal(a.get_xi(), b.get_xi())
+        a.set_tuning([0, 0, 0, 0, 1])
+        a.step(30)
+        for _ in range(30):
+            b.step(1)
+        assert np.array_equal(a.get_xi(), b.get_xi())
+    finally:
+        a.close()
+        b.close()
