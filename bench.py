@@ -130,8 +130,10 @@ def streaming_ceiling(nbytes, reps=200):
 
 
 def load_pmc_traffic(tag, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), or None."""
+    """HBM bytes per launch of the kernel instance `kernel` (its template name,
+    e.g. "k_update<float, 128, 2, false>") from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), or None
+    when that instance was not measured."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{tag}.json")
     if not os.path.exists(path):
         return None
@@ -143,8 +145,31 @@ def load_pmc_traffic(tag, kernel):
         return None
 
 
-KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update": "inv_z_update",
-                "k_plane_fwd": "fwd_yz", "k_plane_upd": "inv_yz_update"}
+def kernel_instance(s, role, tiles):
+    """Template name of the kernel that runs `role` for solver s with the
+    tuning choice `tiles` (fwd z, fwd y, inv y, inv z + update, iteration),
+    as rocprofv3 reports it (sine-series core: the update's GEN flag is false)."""
+    T = "double" if s.precision == pfdd_gl.PFDD_FP64 else "float"
+    nx, ny, nz = s.shape
+    S = s.S
+    if role == "fwd_yz":
+        return f"k_plane_fwd<{T}, {ny}, {nz}>"
+    if role == "inv_yz_update":
+        return f"k_plane_upd<{T}, {ny}, {nz}, false>"
+    if role == "x_contract":
+        if STORED_B:
+            return f"k_mid<{T}, {T}, {nx}, {S}>"
+        return {0: "k_mid_otf", 1: "k_mid_otf_pipe", 2: "k_mid_otf_pl"}[tiles[4]] + f"<{T}, {nx}, {S}>"
+    if role == "inv_z_update":
+        return f"k_update<{T}, {nz}, {tiles[3]}, false>"
+    n, inv, tq = {"fwd_z": (nz, "false", tiles[0]), "fwd_y": (ny, "false", tiles[1]),
+                  "inv_y": (ny, "true", tiles[2])}[role]
+    return f"k_fft_pass<{T}, {n}, {inv}, {tq}>"
+
+
+# kernel families of the GL iteration and the roles they play
+KERNEL_ROLES = {"k_mid_otf": "x_contract", "k_mid<": "x_contract", "k_update<": "inv_z_update",
+                "k_plane_fwd": "fwd_yz", "k_plane_upd": "inv_yz_update", "k_fft_pass": "line_pass"}
 
 
 PROBE_STEPS = 3
@@ -210,26 +235,34 @@ def pmc_summarize(fetch_dir, write_dir, size, out_dir):
     sf = nbytes / (copy_f * 1024.0) if copy_f else 1.0
     sw = nbytes / (copy_w * 1024.0) if copy_w else 1.0
     prob = make_workload(size)
+    short = lambda n: (re.search(r"(k_\w+<[^>]*>)", n) or re.search(r"(.*)", n)).group(1)
     out = {}
     for prec, tag, tname in ((pfdd_gl.PFDD_FP64, "fp64", "double"), (pfdd_gl.PFDD_FP32, "fp32", "float")):
         s = solver_for(prob, prec)
         kbytes, _ = s.traffic()
         s.close()
-        algo = dict(zip(["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"], kbytes.tolist()))
+        # algorithmic bytes per launch: 2 S N e for a line pass or the x pass,
+        # 3 S N e for an update (the plane kernels: 2 and 3 S N e as well)
+        two, three = kbytes[0], kbytes[4]
         res = {"copy_calibration": {"kernel": copy_name, "bytes_read": nbytes, "bytes_written": nbytes,
                                     "fetch_kib": copy_f, "write_kib": copy_w,
                                     "fetch_scale": sf, "write_scale": sw}, "kernels": {}}
-        for sub, role in KERNEL_ROLES.items():
-            m = lambda n, sub=sub: sub in n and f"<{tname}," in n
-            (f, kname), (w, _) = per_kernel(fetch, m, PROBE_STEPS), per_kernel(write, m, PROBE_STEPS)
+        # every instance the probe ran (the tuner's candidates included),
+        # averaged over its last PROBE_STEPS dispatches
+        names = sorted({short(n) for (_, n) in fetch if f"<{tname}," in n and "k_" in n})
+        for kn in names:
+            role = next((r for sub, r in KERNEL_ROLES.items() if kn.startswith(sub)), None)
+            if role is None:
+                continue
+            m = lambda n, kn=kn: short(n) == kn
+            (f, _), (w, _) = per_kernel(fetch, m, PROBE_STEPS), per_kernel(write, m, PROBE_STEPS)
             if f is None or w is None:
                 continue
-            km = re.search(r"(k_\w+<[^>]*>)", kname)
-            res["kernels"][role] = {"kernel": km.group(1) if km else kname,
-                                    "launches": PROBE_STEPS,
-                                    "fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
-                                    "hbm_bytes_per_launch": f * 1024.0 * sf + w * 1024.0 * sw,
-                                    "algorithmic_bytes": algo[role]}
+            algo = three if role in ("inv_z_update", "inv_yz_update") else two
+            res["kernels"][kn] = {"role": role, "launches": PROBE_STEPS,
+                                  "fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
+                                  "hbm_bytes_per_launch": f * 1024.0 * sf + w * 1024.0 * sw,
+                                  "algorithmic_bytes": algo}
         out[tag] = res
         os.makedirs(out_dir, exist_ok=True)
         path = os.path.join(out_dir, f"pmc_traffic_{tag}.json")
@@ -659,9 +692,10 @@ def main():
         s.set_xi(xi_end)     # host -> device
         t_up = time.perf_counter() - t0
         roof = roofline(s)
-        # the committed PMC summary is of the default (on-the-fly) contraction
-        roof["traffic"] = None if STORED_B else load_pmc_traffic(tag, roof["kernel"])
         roof["tiles"] = s.autotune()  # cached choice of the create-time tuning
+        roof["instance"] = kernel_instance(s, roof["kernel"], roof["tiles"])
+        # the committed PMC summary is of the default (on-the-fly) contraction
+        roof["traffic"] = None if STORED_B else load_pmc_traffic(tag, roof["instance"])
         out[tag] = {"sec": dt, "steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                     "host_copy_ms": {"upload": 1e3 * t_up, "download": 1e3 * t_down},
                     "roofline": roof, "e_total": r["e_total"], "xi_end": xi_end}
@@ -718,13 +752,13 @@ def main():
                    "slip_systems": 12, "precision": "FP64 plain (value); FP32 mode below",
                    "contraction": "packed B(k) streamed" if STORED_B else "B(k) evaluated in the x pass"},
         "roofline": {k: f64["roofline"][k] for k in
-                     ("kernel", "bound", "achieved", "peak", "unit", "frac", "traffic")},
+                     ("kernel", "instance", "bound", "achieved", "peak", "unit", "frac", "traffic")},
         "cpu_baseline": cpu,
         "fp32": {"value": f32["steps_per_s"], "ms_per_step": f32["ms_per_step"],
                  "speedup_vs_fp64": f32["steps_per_s"] / f64["steps_per_s"],
                  "rel_err_xi_vs_fp64": err_xi, "rel_err_energy_vs_fp64": err_e,
                  "roofline": {k: f32["roofline"][k] for k in
-                              ("kernel", "achieved", "frac", "traffic")}},
+                              ("kernel", "instance", "achieved", "frac", "traffic")}},
         "step_roofline": {
             "fp64": {"bytes": f64["roofline"]["step_bytes"],
                      "frac": f64["roofline"]["step_bytes"] / (f64["ms_per_step"] * 1e-3) / PEAK_HBM},

@@ -34,6 +34,7 @@
 #include <mutex>
 #include <tuple>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -196,10 +197,12 @@ __global__ void __launch_bounds__(NTHR)
 
 // Deterministic reduction of the block partials in two levels: every block
 // sums a fixed chunk (Kahan within a thread, fixed-shape tree across the
-// block) into stage2; the last block to finish (atomic ticket) sums stage2 in
-// fixed order, writes the iteration's scalars, counts the iteration and
-// raises `done` on convergence or a non-finite result.
+// block) into stage2; the last block to finish (atomic ticket) sums stage2
+// with all its threads (one value per thread and slot, the compensated block
+// tree), writes the iteration's scalars, counts the iteration and raises
+// `done` on convergence or a non-finite result.
 constexpr int NB2 = 32;
+static_assert(NB2 <= NTHR, "the last block reads stage2 one value per thread");
 
 __global__ void __launch_bounds__(NTHR)
     k_reduce(const double *__restrict__ partial, long long pstride, int nb5, int nb3, double invN,
@@ -233,19 +236,15 @@ __global__ void __launch_bounds__(NTHR)
         last = atomicAdd(ticket, 1u) == NB2 - 1;
     }
     __syncthreads();
-    if (!last || tid != 0) return;
+    if (!last) return;
     __threadfence();
+    // one dependent round trip instead of NB2 * NSLOT serial loads in one
+    // thread (the first level's stage2 writes are visible: ticket + fences)
     double w[NSLOT];
-    for (int s = 0; s < NSLOT; ++s) {
-        double sum = 0.0, c = 0.0;
-        for (int b = 0; b < NB2; ++b) {
-            const double y = __builtin_nontemporal_load(&stage2[s * NB2 + b]) - c;
-            const double t = sum + y;
-            c = (t - sum) - y;
-            sum = t;
-        }
-        w[s] = sum;
-    }
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) w[s] = tid < NB2 ? __builtin_nontemporal_load(&stage2[s * NB2 + tid]) : 0.0;
+    block_sum_comp<NSLOT, NTHR>(w, red, tid);
+    if (tid != 0) return;
     *ticket = 0u;
     const double xinorm = sqrt(w[0]);
     const double e_el = 0.5 * w[4] * invN;
@@ -483,6 +482,10 @@ struct pfdd_gl {
         int cur = -1;
         double tol = 0.0;
     } sg;
+    // false when the environment sets PFDD_GL_GRAPH=0 at handle creation:
+    // every iteration is launched eagerly (for tools that cannot follow
+    // graph launches; rocprofv3's kernel trace faults on them here)
+    bool sg_on = true;
     bool plane_ok = false;             // the grid and precision allow the plane iteration
     int nbA = 0, nbC = 0, nb3pl = 0;   // grids of the plane iteration's kernels
     ThetaParams TP{};                  // core flag 5
@@ -1486,9 +1489,13 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     if (rc) return bail(rc);
     e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return bail(fail(PFDD_ERR_HIP, "setup: %s", hipGetErrorString(e)));
+    {
+        const char *g = std::getenv("PFDD_GL_GRAPH");
+        h->sg_on = !(g && g[0] == '0');
+    }
     if (d->flags & PFDD_GL_FLAG_AUTOTUNE) {
         rc = h->prec == PFDD_FP64 ? autotune<double>(h, 4, nullptr) : autotune<float>(h, 4, nullptr);
-        if (!rc) rc = self_graph_ready(h, 0.0);
+        if (!rc && h->sg_on) rc = self_graph_ready(h, 0.0);
         if (rc) return bail(rc);
     }
     *out = h;
@@ -1556,7 +1563,7 @@ int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5])
     rc = h->prec == PFDD_FP64 ? autotune<double>(h, nsteps, choice) : autotune<float>(h, nsteps, choice);
     // the tuned launch shapes are final: capture pfdd_gl_step's graph now
     // (tol 0) rather than inside the caller's first long run
-    return rc ? rc : self_graph_ready(h, 0.0);
+    return rc || !h->sg_on ? rc : self_graph_ready(h, 0.0);
 }
 
 int pfdd_gl_destroy(pfdd_gl *h)
@@ -1651,7 +1658,7 @@ int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out)
     // 6 SG_ITERS), unless the caller is capturing this stream itself
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIPCHK(hipStreamIsCapturing(h->stream, &cs));
-    if (nsteps >= 2 * SG_ITERS && cs == hipStreamCaptureStatusNone) {
+    if (h->sg_on && nsteps >= 2 * SG_ITERS && cs == hipStreamCaptureStatusNone) {
         if ((rc = self_graph_ready(h, tol))) return rc;
         for (int r = 0; r < nsteps / SG_ITERS; ++r) HIPCHK(hipGraphLaunch(h->sg.exec, h->stream));
         h->stepped = true;
