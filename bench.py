@@ -606,6 +606,8 @@ def bench_frank_read(args):
                              precision=prec)
         s.set_xi(prob["xi0"])
         s.set_tau(prob["tau"])
+        # tile shapes and iteration variant for this grid (restores the field)
+        tiles = s.autotune()
         s.step(0)
         t0 = time.perf_counter()
         done = 0
@@ -615,11 +617,16 @@ def bench_frank_read(args):
             done += k
             print(f"frank-read {tag}: {done} steps", file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
-        step_bytes = s.traffic()[1]
+        kbytes, step_bytes = s.traffic()
+        xs[tag] = s.get_xi()
+        # per-kernel device times (HIP events recorded by the dispatches)
+        prof = s.profile(20)
+        slots = ["fwd_z", "fwd_y", "x_contract", "inv_y", "inv_z_update"]
+        per_kernel = {k: {"ms": prof[k], "GBps": kbytes[q] / (prof[k] * 1e-3) / 1e9}
+                      for q, k in enumerate(slots) if kbytes[q] > 0}
         out[tag] = {"steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                     "hbm_GBps": step_bytes * args.steps / dt / 1e9, "bytes_per_step": step_bytes,
-                    "e_total": r["e_total"], "xiave": r["xiave"]}
-        xs[tag] = s.get_xi()
+                    "e_total": r["e_total"], "xiave": r["xiave"], "tiles": tiles, "per_kernel": per_kernel}
         s.close()
     err = float(np.max(np.abs(xs["fp32"] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
     e_err = abs(out["fp32"]["e_total"] - out["fp64"]["e_total"]) / abs(out["fp64"]["e_total"])

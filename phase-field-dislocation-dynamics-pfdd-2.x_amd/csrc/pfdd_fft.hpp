@@ -255,18 +255,20 @@ constexpr int pick_radix(int m)
     return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m % 5 == 0 ? 5 : m;
 }
 
-// Lines per LDS tile for a transform of length N: a power of two giving about
-// 32 KiB of complex data per tile.
 // Lines per LDS tile of a line pass: the largest power of two whose tile
 // holds at most (1024 << TQ) FP64 or (2048 << TQ) FP32 elements, at least one
-// line, and whose padded tile fits 60 KiB of LDS.  TQ is the tile-shape
-// choice of the autotuner (0, 1, 2).
+// line, and whose padded tile fits 60 KiB of LDS (TQ 0, 1) or 80 KiB (TQ 2:
+// two blocks per CU of the 160 KiB).  TQ is the tile-shape choice of the
+// autotuner (0, 1, 2).  The larger cap matters for long lines, whose strided
+// passes move runs of NL elements: 512-point FP64 lines get 8 lines (128-byte
+// runs) instead of 4.
 template <typename R, int N, int TQ> constexpr int tile_lines_q()
 {
     constexpr int target = (sizeof(R) == 8 ? 1024 : 2048) << TQ;
+    constexpr long long cap = TQ == 2 ? 81920 : 61440;
     int nl = 1;
     while (nl * 2 * N <= target) nl *= 2;
-    while (nl > 1 && (long long)N * (nl + 1) * 2 * (int)sizeof(R) > 61440) nl /= 2;
+    while (nl > 1 && (long long)N * (nl + 1) * 2 * (int)sizeof(R) > cap) nl /= 2;
     return nl;
 }
 
