@@ -239,7 +239,10 @@ int pfdd_gl_theta(pfdd_gl *h, double *theta);
 /* Autotuner (no counterpart in the reference): times each candidate LDS tile
  * shape (0, 1, 2: about 1K/2K/4K FP64 or 2K/4K/8K FP32 elements per tile) of
  * the four line passes -- forward z, forward y, inverse y, inverse z + update
- * -- and the iteration variant of choice[4]:
+ * -- plus, for the first three, the FFT radix plan: choice 3, 4, 5 are shapes
+ * 0, 1, 2 with radix-16 stages wherever they save a stage (lengths 16, 48,
+ * 80, 128, 256: 128 = 16 x 8 instead of 8 x 8 x 2); 0-2 use radix 8 first.
+ * choice[4] is the iteration variant:
  *   0  five kernels, x pass one tile per workgroup;
  *   1  five kernels, x pass on persistent workgroups that load the next tile
  *      while transforming the current one;
@@ -249,12 +252,14 @@ int pfdd_gl_theta(pfdd_gl *h, double *theta);
  *      16..128, on-the-fly B.
  * Timing runs on this handle's own buffers: after 20 warm-up iterations, two
  * interleaved rounds of `nsteps` GL iterations per candidate, keeping each
- * kernel's best time; the fastest shape per line pass and the faster x-pass
- * variant win, and variant 2 replaces them if its whole iteration is faster
- * (choice[5], may be NULL).  The field and step counters are restored; the
- * choice is cached for the process per grid, system count, batch, precision
- * and options.  Results (xi) do not depend on the choice.  The update of
- * core models 2, 4 and 9 always uses tile shape 0. */
+ * kernel's best time; the fastest shape and plan per line pass and the
+ * faster x-pass variant win, and variant 2 replaces them if its whole
+ * iteration is faster (choice[5], may be NULL).  The field and step counters
+ * are restored; the choice is cached for the process per grid, system count,
+ * batch, precision and options.  Tile shapes and iteration variants give the
+ * same xi bit for bit; the two radix plans round differently (relative
+ * differences of a few ulp per transform).  The update of core models 2, 4
+ * and 9 always uses tile shape 0. */
 int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5]);
 /* Sets the choices explicitly (same meaning as pfdd_gl_autotune's output). */
 int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5]);

@@ -340,6 +340,11 @@ constexpr int pick_radix_x(int m)
     return (m % 16 == 0 && 1 + stages_x(m / 16) < stages_r8(m)) ? 16 : pick_radix(m);
 }
 
+// The line passes offer both radix plans to the autotuner where they differ
+// (lengths 16, 48, 80, 128, 256): plan 0 the radix-8 plan of fft_stages,
+// plan 1 the radix-16 plan of fft_stages_x.
+template <int N> constexpr bool radix16_plan_differs() { return stages_x(N) < stages_r8(N); }
+
 // The same transform with those radix-16 stages (128 = 16 x 8: two LDS round
 // trips instead of three).  Used by the x pass, whose tiles hold all S
 // systems of a column: there the saved LDS traffic outweighs the 16 values a

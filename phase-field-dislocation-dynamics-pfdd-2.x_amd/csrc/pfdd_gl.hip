@@ -744,10 +744,12 @@ struct TuneKey {
 std::mutex g_tune_mu;
 std::map<TuneKey, std::array<int, 5>> g_tune_cache;
 
-// Times every tile shape (0, 1, 2) of each line pass and both x-pass variants
-// and keeps the fastest per kernel.  The field, ping-pong parity and step
-// counters are restored.  Choices that would break the reverse mode's
-// one-system-per-block rule for the update pass are skipped.
+// Times every tile shape (0, 1, 2) of each line pass, with both radix plans
+// where the pass length has two (tq 3-5: the shapes with the radix-16 plan),
+// and the x-pass / iteration variants, and keeps the fastest per kernel.  The
+// field, ping-pong parity and step counters are restored.  Choices that would
+// break the reverse mode's one-system-per-block rule for the update pass are
+// skipped.
 template <typename R>
 int autotune(pfdd_gl *h, int nsteps, int choice[5])
 {
@@ -771,17 +773,23 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
     HIPCHK(hipMemcpyAsync(save, h->xi[cur0], fb, hipMemcpyDeviceToDevice, h->stream));
     int arg[5] = {1, 1, 1, 1, 0};
     int rc = PFDD_OK;
-    // candidates: tile shape q = 0, 1, 2 with the one-tile-per-block x pass,
-    // then (on-the-fly contraction only) the pipelined x pass, then (where
-    // the grid allows it) the three-kernel plane iteration.  The clock of an
-    // idle GPU ramps up over the first few dozen dispatches, so all
-    // candidates are timed after a common warm-up, in two interleaved rounds,
-    // keeping each kernel's best time per candidate.  The five-kernel
-    // iteration takes the best shape of each kernel; the plane iteration
-    // replaces it if its whole step (three kernels) is faster than that sum.
-    const int ncand = h->stored_B ? 3 : h->plane_ok ? 5 : 4;
-    double tmin[5][5];
-    for (int c = 0; c < 5; ++c)
+    // candidates: tile shape q = 0, 1, 2 with the one-tile-per-block x pass;
+    // the same shapes with the radix-16 plan in the line passes (q + 3,
+    // where the z or y length has that plan); then (on-the-fly contraction
+    // only) the pipelined x pass, then (where the grid allows it) the
+    // three-kernel plane iteration.  The clock of an idle GPU ramps up over
+    // the first few dozen dispatches, so all candidates are timed after a
+    // common warm-up, in two interleaved rounds, keeping each kernel's best
+    // time per candidate.  The five-kernel iteration takes the best shape and
+    // plan of each kernel; the plane iteration replaces it if its whole step
+    // (three kernels) is faster than that sum.
+    const bool plan_z = radix16_plan_differs_rt(h->nz), plan_y = radix16_plan_differs_rt(h->ny);
+    const int nshape = (plan_z || plan_y) ? 6 : 3;
+    const int c_pipe = h->stored_B ? -1 : nshape;
+    const int c_plane = (!h->stored_B && h->plane_ok) ? nshape + 1 : -1;
+    const int ncand = nshape + (c_pipe >= 0 ? 1 : 0) + (c_plane >= 0 ? 1 : 0);
+    double tmin[8][5];
+    for (int c = 0; c < 8; ++c)
         for (int p = 0; p < 5; ++p) tmin[c][p] = 1e300;
     {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
@@ -790,7 +798,9 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
     }
     for (int round = 0; round < 2 && !rc; ++round)
         for (int c = 0; c < ncand && !rc; ++c) {
-            const int all[5] = {c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? c : 1, c < 3 ? 0 : c - 2};
+            const int ln = c < nshape ? c : 1, up = c < nshape ? c % 3 : 1;
+            const int var = c < nshape ? 0 : c == c_pipe ? 1 : 2;
+            const int all[5] = {ln, ln, ln, up, var};
             set_tiles(h, all);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
             double warm[5], t[5];
@@ -799,23 +809,26 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
         }
     if (!rc) {
         double best[5] = {1e300, 1e300, 1e300, 1e300, 1e300};
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < nshape; ++q) {
             const int nl = h->prec == PFDD_FP64 ? tile_lines_rt<double>(h->nz, q) : tile_lines_rt<float>(h->nz, q);
             const bool upd_ok = ((long long)h->nx * h->ny) % nl == 0;
+            // a plan-1 candidate counts for a line pass only where its length
+            // has a distinct radix-16 plan; the update takes shapes 0-2
+            const bool use[4] = {q < 3 || plan_z, q < 3 || plan_y, q < 3 || plan_y, q < 3 && upd_ok};
             for (int p = 0; p < 4; ++p)
-                if (tmin[q][p] < best[p] && (p < 3 || upd_ok)) {
+                if (use[p] && tmin[q][p] < best[p]) {
                     best[p] = tmin[q][p];
                     arg[p] = q;
                 }
             best[4] = tmin[q][4] < best[4] ? tmin[q][4] : best[4];
         }
-        if (ncand >= 4 && tmin[3][4] < best[4]) {
-            best[4] = tmin[3][4];
+        if (c_pipe >= 0 && tmin[c_pipe][4] < best[4]) {
+            best[4] = tmin[c_pipe][4];
             arg[4] = 1;
         }
-        if (ncand == 5) {
+        if (c_plane >= 0) {
             const double five = best[0] + best[1] + best[2] + best[3] + best[4];
-            const double three = tmin[4][0] + tmin[4][3] + tmin[4][4];
+            const double three = tmin[c_plane][0] + tmin[c_plane][3] + tmin[c_plane][4];
             if (three < five) arg[4] = 2;
         }
     }
@@ -1541,8 +1554,10 @@ int pfdd_gl_set_tuning(pfdd_gl *h, const int choice[5])
     int rc;
     if ((rc = check_handle(h))) return rc;
     if (!choice) return fail(PFDD_ERR_ARG, "null choice");
-    for (int p = 0; p < 4; ++p)
-        if (choice[p] < 0 || choice[p] > 2) return fail(PFDD_ERR_ARG, "tile shape choice must be 0, 1 or 2");
+    for (int p = 0; p < 3; ++p)
+        if (choice[p] < 0 || choice[p] > 5)
+            return fail(PFDD_ERR_ARG, "line-pass choice must be 0-5 (tile shape + 3 x radix plan)");
+    if (choice[3] < 0 || choice[3] > 2) return fail(PFDD_ERR_ARG, "update tile shape choice must be 0, 1 or 2");
     if (choice[4] < 0 || choice[4] > 2) return fail(PFDD_ERR_ARG, "iteration choice must be 0, 1 or 2");
     if (choice[4] == 2 && !h->plane_ok)
         return fail(PFDD_ERR_UNSUPPORTED, "the plane iteration needs square y-z planes that fit in LDS "

@@ -248,11 +248,22 @@ inline bool size_supported(int n)
     return false;
 }
 
+// Whether the line passes have a distinct radix-16 plan for length n.
+inline bool radix16_plan_differs_rt(int n)
+{
+#define PFDD_CASE(n_) if (n == n_) return radix16_plan_differs<n_>();
+    PFDD_SIZES(PFDD_CASE)
+#undef PFDD_CASE
+    return false;
+}
+
 inline bool systems_supported(int s) { return s == 1 || s == 2 || s == 3 || s == 4 || s == 6 || s == 12; }
 
+// Lines per tile of tile shape tq % 3 (the radix plan does not change it).
 template <typename R>
 int tile_lines_rt(int n, int tq)
 {
+    tq %= 3;
 #define PFDD_CASE(n_)                                              \
     if (n == n_) {                                                 \
         if (tq == 0) return tile_lines_q<R, n_, 0>();              \
@@ -277,7 +288,9 @@ int tile_lines_rt(int n, int tq)
 // grid size (or system count) outside the dispatch table.
 
 // Line pass (pfdd_pass.hip): FFT along one axis of nlines lines, input line
-// map mi, output line map mo; zmask (optional) zeroes input elements.
+// map mi, output line map mo; zmask (optional) zeroes input elements.  tq =
+// tile shape (0, 1, 2) + 3 * radix plan (0 radix 8 first, 1 radix 16 where it
+// saves a stage).
 template <typename R, bool INV>
 hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo, long long nlines,
                        const void *tw, const int *done, hipStream_t st, const uint8_t *zmask = nullptr,

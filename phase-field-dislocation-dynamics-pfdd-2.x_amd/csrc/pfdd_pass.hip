@@ -5,7 +5,9 @@
 namespace pfdd {
 
 // ------------------------------------------------- generic FFT pass (1, 2, 4)
-template <typename R, int N, bool INV, int TQ>
+// TQ: tile shape (tile_lines_q); PL: radix plan (0 radix 8 first, 1 radix 16
+// where it saves a stage).
+template <typename R, int N, bool INV, int TQ, int PL>
 __global__ void __launch_bounds__(NTHR)
     k_fft_pass(const typename cvec<R>::type *__restrict__ in, typename cvec<R>::type *__restrict__ out,
                LineMap mi, LineMap mo, long long nlines, const typename cvec<R>::type *__restrict__ tw,
@@ -36,7 +38,8 @@ __global__ void __launch_bounds__(NTHR)
         }
         __syncthreads();
     }
-    fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
+    if constexpr (PL == 1) fft_stages_x<N, 1, NL, NTHR, INV>(lds, tw, tid);
+    else fft_stages<N, 1, NL, NTHR, INV>(lds, tw, tid);
     tile_store<N, NL, NTHR>(out, lds, bout, mo.SP, tid);
 }
 
@@ -47,13 +50,20 @@ hipError_t launch_pass(int n, const void *in, void *out, LineMap mi, LineMap mo,
                        int tq)
 {
     typedef typename cvec<R>::type V;
+    // tq = shape + 3 * plan; plan 1 falls back to plan 0 where they coincide
+    const int shape = tq % 3, plan = tq / 3;
 #define PFDD_CASE(n_)                                                                               \
     if (n == n_) {                                                                                  \
-        PFDD_TQ_DISPATCH(tq, {                                                                      \
+        PFDD_TQ_DISPATCH(shape, {                                                                   \
             constexpr int NL = tile_lines_q<R, n_, TQ>();                                           \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                 \
-            PFDD_LAUNCH((k_fft_pass<R, n_, INV, TQ>), dim3(nb), dim3(NTHR), st,                     \
-                               (const V *)in, (V *)out, mi, mo, nlines, (const V *)tw, done, zmask); \
+            if (plan == 1 && radix16_plan_differs<n_>())                                            \
+                PFDD_LAUNCH((k_fft_pass<R, n_, INV, TQ, radix16_plan_differs<n_>() ? 1 : 0>),      \
+                            dim3(nb), dim3(NTHR), st, (const V *)in, (V *)out, mi, mo, nlines,      \
+                            (const V *)tw, done, zmask);                                            \
+            else                                                                                    \
+                PFDD_LAUNCH((k_fft_pass<R, n_, INV, TQ, 0>), dim3(nb), dim3(NTHR), st,              \
+                            (const V *)in, (V *)out, mi, mo, nlines, (const V *)tw, done, zmask);   \
         })                                                                                          \
         return hipGetLastError();                                                                   \
     }

@@ -131,6 +131,7 @@ def test_autotune_keeps_results(precision):
     s1.step(3)
     x_before = s1.get_xi()
     choice = s1.autotune(3)
+    # 32-point lines have one radix plan, so the line-pass choices stay 0-2
     assert len(choice) == 5 and all(q in (0, 1, 2) for q in choice[:4]) and choice[4] in (0, 1, 2)
     assert np.array_equal(s1.get_xi(), x_before)  # tuning restores the field
     s0.step(3)
@@ -147,8 +148,13 @@ def test_autotune_keeps_results(precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,shape,systems", [(0, (32, 32, 32), "basal3"), (1, (32, 32, 32), "basal3"),
-                                                     (0, (16, 16, 16), "fcc12"), (1, (48, 32, 16), "basal3")])
+                                                     (0, (16, 16, 16), "fcc12"), (1, (48, 32, 16), "basal3"),
+                                                     (0, (16, 128, 128), "basal3"), (1, (16, 128, 128), "basal3")])
 def test_every_tuning_choice_gives_the_same_field(precision, shape, systems):
+    """Tile shapes and iteration variants: the same field bit for bit.  The
+    radix-16 plan of the line passes (choices 3-5; lengths 16, 48, 128 here
+    have one) rounds differently: within 1e-12 (FP64) / 1e-5 (FP32) of the
+    radix-8 plan after 12 iterations, and bit for bit across tile shapes."""
     print(f"case: tuning choices {shape} {systems} precision {precision}", flush=True)
     c = P.make_case(shape=shape, systems=systems, noise=1e-2, sigma_mu=0.004)
     ref = P.gpu_solver(c, precision=precision)
@@ -156,13 +162,29 @@ def test_every_tuning_choice_gives_the_same_field(precision, shape, systems):
     ref.step(12)
     x_ref = ref.get_xi()
     ref.close()
-    plane = shape[1] == shape[2]  # the three-kernel plane iteration needs square y-z planes
+    # the three-kernel plane iteration needs square y-z planes that fit in LDS
+    plane = shape[1] == shape[2] and shape[1] <= (80 if precision == 0 else 128)
     for choice in ([0, 0, 0, 0, 0], [2, 2, 2, 2, 0], [1, 1, 1, 1, 1], [0, 2, 1, 0, 1]) + (([1, 1, 1, 1, 2],) if plane else ()):
         s = P.gpu_solver(c, precision=precision)
         s.set_tuning(choice)
         s.step(12)
         assert np.array_equal(s.get_xi(), x_ref), choice
         s.close()
+    x_p1 = None
+    tol = 1e-12 if precision == 0 else 1e-5
+    for choice in ([3, 3, 3, 1, 0], [5, 4, 3, 0, 1]):
+        s = P.gpu_solver(c, precision=precision)
+        s.set_tuning(choice)
+        s.step(12)
+        x = s.get_xi()
+        s.close()
+        err = float(np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref)))
+        print(f"radix-16 plan {choice}: rel. difference from the radix-8 plan {err:.3e}", flush=True)
+        assert err < tol, (choice, err)
+        if x_p1 is None:
+            x_p1 = x
+        else:
+            assert np.array_equal(x, x_p1), choice
 
 
 @pytest.mark.gpu
