@@ -262,15 +262,20 @@ constexpr int pick_radix(int m)
 // autotuner (0, 1, 2).  The larger cap matters for long lines, whose strided
 // passes move runs of NL elements: 512-point FP64 lines get 8 lines (128-byte
 // runs) instead of 4.
-template <typename R, int N, int TQ> constexpr int tile_lines_q()
+template <typename R, int N, int TQ, long long CAP = (TQ == 2 ? 81920 : 61440)> constexpr int tile_lines_q()
 {
     constexpr int target = (sizeof(R) == 8 ? 1024 : 2048) << TQ;
-    constexpr long long cap = TQ == 2 ? 81920 : 61440;
     int nl = 1;
     while (nl * 2 * N <= target) nl *= 2;
-    while (nl > 1 && (long long)N * (nl + 1) * 2 * (int)sizeof(R) > cap) nl /= 2;
+    while (nl > 1 && (long long)N * (nl + 1) * 2 * (int)sizeof(R) > CAP) nl /= 2;
     return nl;
 }
+
+// Lines per tile of the update kernels (inverse z pass + update, and its
+// adjoint): the same shapes with the 60 KiB cap for all three.  They hold
+// per-thread state for every cell of the tile they update (old values, sums),
+// and 80 KiB tiles (up to 32 cells per thread) measured 30-40 % slower.
+template <typename R, int N, int TQ> constexpr int tile_lines_u() { return tile_lines_q<R, N, TQ, 61440>(); }
 
 template <typename R, int N> constexpr int tile_lines() { return tile_lines_q<R, N, 1>(); }
 

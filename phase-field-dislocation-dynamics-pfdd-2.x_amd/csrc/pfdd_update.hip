@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(NTHR)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int NL = tile_lines_u<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     PFDD_TILE(V, lds, N * LDP);
     __shared__ long long bin[NL], bout[NL];
     // per-line tau and core coefficients: read once per block here, since
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(NTHR)
 {
     typedef typename cvec<R>::type V;
     if (*done) return;
-    constexpr int NL = tile_lines_q<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
+    constexpr int NL = tile_lines_u<R, N, TQ>(), LDP = NL + 1, TOT = N * NL;
     constexpr int PER = (TOT + NTHR - 1) / NTHR;
     constexpr int NG = 1 + MAXM;
     PFDD_TILE(V, lds, N * LDP);
@@ -175,8 +175,7 @@ __global__ void __launch_bounds__(NTHR)
         const R A = amp ? (R)amp[idx] : (R)1;
         // sin, cos of 2 pi m x by the angle-addition recurrence from m = 1
         R sp, cp;
-        if constexpr (sizeof(R) == 8) sincospi((double)x.x, (double *)&sp, (double *)&cp);
-        else sincospif((float)x.x, (float *)&sp, (float *)&cp);
+        sincospi_poly(x.x, sp, cp);
         const R s1 = (R)2 * sp * cp, c1 = cp * cp - sp * sp;
         R sm = s1, cm = c1, d2 = (R)0, hv = (R)0;
         if (P.model == PFDD_CORE_EXTENDED) {
@@ -255,7 +254,7 @@ hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn
 #define PFDD_CASE(n_)                                                                                \
     if (n == n_) {                                                                                   \
         if (gen) {                                                                                   \
-            constexpr int NL = tile_lines_q<R, n_, 0>();                                             \
+            constexpr int NL = tile_lines_u<R, n_, 0>();                                             \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
             PFDD_LAUNCH((k_update<R, n_, 0, true>), dim3(nb), dim3(NTHR), st, (const V *)T1,         \
                         (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, P,  \
@@ -263,7 +262,7 @@ hipError_t launch_update(int n, int tq, const void *T1, const void *xo, void *xn
             return hipGetLastError();                                                                \
         }                                                                                            \
         PFDD_TQ_DISPATCH(tq, {                                                                       \
-            constexpr int NL = tile_lines_q<R, n_, TQ>();                                            \
+            constexpr int NL = tile_lines_u<R, n_, TQ>();                                            \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                  \
             PFDD_LAUNCH((k_update<R, n_, TQ, false>), dim3(nb), dim3(NTHR), st, (const V *)T1,       \
                                (const V *)xo, (V *)xn, mask, prev, (const R *)amp, mi, mo, nlines, lps, \
@@ -286,7 +285,7 @@ hipError_t launch_adj_update(int n, int tq, const void *T1, const void *lam, con
 #define PFDD_CASE(n_)                                                                                  \
     if (n == n_) {                                                                                     \
         PFDD_TQ_DISPATCH(tq, {                                                                         \
-            constexpr int NL = tile_lines_q<R, n_, TQ>();                                              \
+            constexpr int NL = tile_lines_u<R, n_, TQ>();                                              \
             const unsigned nb = (unsigned)((nlines + NL - 1) / NL);                                    \
             hipLaunchKernelGGL((k_adj_update<R, n_, TQ>), dim3(nb), dim3(NTHR), 0, st, (const V *)T1,  \
                                (const V *)lam, (const V *)xs, (V *)lam_out, mask, (const R *)amp, mi, mo, \
