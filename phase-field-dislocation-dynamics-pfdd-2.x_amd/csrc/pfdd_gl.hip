@@ -734,6 +734,8 @@ int time_passes(pfdd_gl *h, int nsteps, double acc[5])
     return rc;
 }
 
+int self_graph_ready(pfdd_gl *h, double tol);
+
 struct TuneKey {
     int nx, ny, nz, S, M, prec, flags;
     bool operator<(const TuneKey &o) const
@@ -766,11 +768,24 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
         }
     }
     const size_t fb = (size_t)h->SM * h->N * h->esz;
-    void *save = nullptr;
-    HIPCHK(hipMalloc(&save, fb));
     const int cur0 = h->cur;
     const bool stepped0 = h->stepped;
+    // the current field, and (after a step) the previous one, which the
+    // output-step strain of the reference is computed from (Q2)
+    const int nsave = stepped0 ? 2 : 1;
+    char *save = nullptr;
+    HIPCHK(hipMalloc((void **)&save, nsave * fb));
     HIPCHK(hipMemcpyAsync(save, h->xi[cur0], fb, hipMemcpyDeviceToDevice, h->stream));
+    if (stepped0) HIPCHK(hipMemcpyAsync(save + fb, h->xi[1 - cur0], fb, hipMemcpyDeviceToDevice, h->stream));
+    auto restore = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(h->xi[cur0], save, fb, hipMemcpyDeviceToDevice, h->stream));
+        if (stepped0) HIPCHK(hipMemcpyAsync(h->xi[1 - cur0], save + fb, fb, hipMemcpyDeviceToDevice, h->stream));
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
+        HIPCHK(hipGetLastError());
+        h->cur = cur0;
+        h->stepped = stepped0;
+        return PFDD_OK;
+    };
     int arg[5] = {1, 1, 1, 1, 0};
     int rc = PFDD_OK;
     // candidates: tile shape q = 0, 1, 2 with the one-tile-per-block x pass;
@@ -832,18 +847,29 @@ int autotune(pfdd_gl *h, int nsteps, int choice[5])
             if (three < five) arg[4] = 2;
         }
     }
-    HIPCHK(hipMemcpyAsync(h->xi[cur0], save, fb, hipMemcpyDeviceToDevice, h->stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, h->done, h->steps);
-    HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipFree(save));
-    h->cur = cur0;
-    h->stepped = stepped0;
+    int rc2 = restore();
+    if (!rc && !rc2) {
+        set_tiles(h, arg);
+        // the tuned launch shapes are final: capture pfdd_gl_step's graph now
+        // (tol 0) rather than inside the caller's first long run, replay it
+        // once from the restored field (which also keeps the GPU busy up to
+        // the caller's first run instead of idling through the capture) and
+        // restore again
+        if (h->sg_on && !(rc = self_graph_ready(h, 0.0))) {
+            const hipError_t e = hipGraphLaunch(h->sg.exec, h->stream);
+            if (e != hipSuccess) rc = fail(PFDD_ERR_HIP, "autotune graph replay: %s", hipGetErrorString(e));
+            rc2 = restore();
+        }
+    }
+    const hipError_t es = hipStreamSynchronize(h->stream);
+    (void)hipFree(save);
+    if (!rc) rc = rc2;
+    if (!rc && es != hipSuccess) rc = fail(PFDD_ERR_HIP, "autotune: %s", hipGetErrorString(es));
     if (rc) {
         const int dflt[5] = {1, 1, 1, 1, 0};
         set_tiles(h, dflt);
         return rc;
     }
-    set_tiles(h, arg);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = {arg[0], arg[1], arg[2], arg[3], arg[4]};
@@ -1508,6 +1534,8 @@ int pfdd_gl_create_batch(const pfdd_gl_desc *d, int nbatch, pfdd_gl **out)
     }
     if (d->flags & PFDD_GL_FLAG_AUTOTUNE) {
         rc = h->prec == PFDD_FP64 ? autotune<double>(h, 4, nullptr) : autotune<float>(h, 4, nullptr);
+        // (a cached choice returns without tuning: capture here; a no-op
+        // when the tuner has just captured)
         if (!rc && h->sg_on) rc = self_graph_ready(h, 0.0);
         if (rc) return bail(rc);
     }
@@ -1576,8 +1604,7 @@ int pfdd_gl_autotune(pfdd_gl *h, int nsteps, int choice[5])
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nsteps < 1) return fail(PFDD_ERR_ARG, "nsteps must be >= 1");
     rc = h->prec == PFDD_FP64 ? autotune<double>(h, nsteps, choice) : autotune<float>(h, nsteps, choice);
-    // the tuned launch shapes are final: capture pfdd_gl_step's graph now
-    // (tol 0) rather than inside the caller's first long run
+    // a cached choice returns without tuning: its graph may still be stale
     return rc || !h->sg_on ? rc : self_graph_ready(h, 0.0);
 }
 

@@ -144,8 +144,11 @@ template <typename T> struct GConst {
     T inv2C44, inv1mnu, invmu, C44x2, C12;
 };
 
-// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im),
-// computed in T (double in the FP64 path, float in the FP32 mode).
+// Hands ghat_a for a = H*SA .. H*SA+SA-1 of one k-point to sink(aa, re, im,
+// xihat_a), computed in T (double in the FP64 path, float in the FP32 mode).
+// xihat_a is the value read for the contraction, kept in registers so that
+// the sink does not read the tile again (FP64 x pass at 128^3 x 12: 189 ->
+// 186 us).
 template <typename T, int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
 __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const T f[3], bool nzf, T c2, T c4,
                                           const T *__restrict__ cst, const GConst<T> &G, Sink sink)
@@ -154,9 +157,11 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
     // hoisted and held in registers across points and systems
     __asm__ volatile("" ::: "memory");
     T Pr[6] = {}, Pi[6] = {}; // Sig, then P
+    V xk[SA];
 #pragma unroll
     for (int b = 0; b < S; ++b) {
         const V v = lds.ld(p * LDP + b * NP + q);
+        if (b >= H * SA && b < H * SA + SA) xk[b - H * SA] = v;
         const T xr = (T)v.x, xi = (T)v.y;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -202,7 +207,7 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
         gr *= G.invmu;
         gi *= G.invmu;
         if (!nzf) gr = gi = (T)0;
-        sink(aa, gr, gi);
+        sink(aa, gr, gi, xk[aa]);
     }
 }
 
@@ -278,9 +283,8 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
             T f[3], c2, c4;
             bool nzf;
             otf_freq<N, NP, S, T, PL>(G, c0 + q, p, f, nzf, c2, c4);
-            otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, [&](int a, T gr, T gi) {
+            otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, [&](int a, T gr, T gi, V x) {
                 const int idx = p * LDP + a * NP + q;
-                const V x = lds.ld(idx);
                 el += (double)x.x * gr + (double)x.y * gi;
                 lds.st(idx, mk<V>((R)gr, (R)gi));
             });
@@ -295,7 +299,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
         bool nzf;
         otf_freq<N, NP, S, T, PL>(G, c0 + q, p, f, nzf, c2, c4);
         T g[SA][2];
-        auto keep = [&](int aa, T gr, T gi) {
+        auto keep = [&](int aa, T gr, T gi, V) {
             g[aa][0] = gr;
             g[aa][1] = gi;
         };

@@ -130,10 +130,17 @@ def test_autotune_keeps_results(precision):
     s1 = P.gpu_solver(c, precision=precision)
     s1.step(3)
     x_before = s1.get_xi()
+    sig = np.diag([0.0, 0.0, 0.0]) + 1e-3 * np.eye(3)
+    # the output-step strain is computed from the field before the last update
+    # (the other ping-pong buffer): tuning restores both
+    f_before = s1.diag_fields(sig)
     choice = s1.autotune(3)
     # 32-point lines have one radix plan, so the line-pass choices stay 0-2
     assert len(choice) == 5 and all(q in (0, 1, 2) for q in choice[:4]) and choice[4] in (0, 1, 2)
     assert np.array_equal(s1.get_xi(), x_before)  # tuning restores the field
+    f_after = s1.diag_fields(sig)
+    for k, (a, b) in enumerate(zip(f_before, f_after)):
+        assert np.array_equal(a, b), k
     s0.step(3)
     s0.step(17)
     s1.step(17)
