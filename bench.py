@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import subprocess
 import math
 import os
 import re
@@ -100,33 +99,6 @@ def roofline(s, prof_steps=50, warm_steps=50):
         "field_bytes": kbytes[0] / 2,
         "iteration": "three-kernel plane" if plane else "five-kernel",
     }
-
-
-def streaming_ceiling(nbytes, reps=200):
-    """Practical HBM streaming rate of this GPU for the field size of one pass:
-    torch's element-wise kernels over FP64 tensors of nbytes each, timed with
-    events -- mul(x, 1, out=y) moves the bytes of a line pass (one read, one
-    write), add_ those of the update (two reads, one write)."""
-    import torch
-    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    y = torch.ones_like(x)
-    res = {}
-    for name, fn, moved in (("mul (1 read + 1 write)", lambda: torch.mul(x, 1.0, out=y), 2 * nbytes),
-                            ("add_ (2 reads + 1 write)", lambda: y.add_(x), 3 * nbytes)):
-        for _ in range(20):
-            fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        res[name] = {"us": 1e3 * ms, "bytes": moved, "GBps": moved / (ms * 1e-3) / 1e9}
-    del x, y
-    torch.cuda.empty_cache()
-    return res
 
 
 def load_pmc_traffic(tag, kernel):
@@ -656,8 +628,6 @@ def main():
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="turn two rocprofv3 --pmc outputs into pmc_traffic_*.json")
     ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles"))
-    ap.add_argument("--streaming-ceiling", type=int, default=0, metavar="BYTES",
-                    help="print torch's element-wise streaming rates over FP64 tensors of BYTES each (JSON)")
     ap.add_argument("--workload", default="c1", choices=["c1", "ensemble", "depinning", "adjoint", "frank-read"],
                     help="c1: the 128^3 x 12 headline (default); the others print their own JSON line")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="ensemble precision")
@@ -675,9 +645,6 @@ def main():
               "the multi-GPU slab decomposition is a design note, not built.", file=sys.stderr)
         return 2
 
-    if args.streaming_ceiling:
-        print(json.dumps(streaming_ceiling(args.streaming_ceiling)))
-        return 0
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "depinning":
@@ -715,28 +682,24 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(prob, args.cpu_steps)
-    # the practical streaming ceiling the passes are compared with (the
-    # roofline peak stays the nominal 8 TB/s)
-    # measured in a child process: torch brings its own HIP runtime, which
-    # cannot initialise next to the one the solver library already uses here
-    # (per precision: the FP32 field is half the size, and part of it stays in
-    # the last-level cache between passes)
-    # A failed probe costs only this context field, never the bench line.
+    # the practical streaming rates the passes are compared with (the
+    # roofline peak stays the nominal 8 TB/s): the library's own nontemporal
+    # copy and add kernels over arrays of one field each (per precision: the
+    # FP32 field is half the size, and part of it stays in the last-level
+    # cache between passes).  A failed probe costs only this context field.
     ceil = {}
     for tag, o in out.items():
         nb = int(o["roofline"]["field_bytes"])
         try:
-            child = subprocess.run([sys.executable, os.path.abspath(__file__), "--streaming-ceiling", str(nb)],
-                                   capture_output=True, text=True, timeout=600)
-            if child.returncode != 0:
-                raise RuntimeError(child.stderr[-2000:])
-            ceil[tag] = json.loads(child.stdout.strip().splitlines()[-1])
+            cp, ad = pfdd_gl.stream_rates(nb)
         except Exception as exc:  # noqa: BLE001
-            print(f"bench.py: streaming-ceiling probe ({tag}) failed: {exc}", file=sys.stderr)
+            print(f"bench.py: streaming-rate probe ({tag}) failed: {exc}", file=sys.stderr)
             ceil[tag] = None
             continue
-        rate = {"line": ceil[tag]["mul (1 read + 1 write)"]["GBps"],
-                "update": ceil[tag]["add_ (2 reads + 1 write)"]["GBps"]}
+        ceil[tag] = {"copy (1 read + 1 write)": {"GBps": cp, "bytes": 2 * nb},
+                     "add (2 reads + 1 write)": {"GBps": ad, "bytes": 3 * nb},
+                     "kernels": "pfdd_gl_stream_rates: nontemporal 16-byte loads and stores, one element per thread"}
+        rate = {"line": cp, "update": ad}
         for role, k in o["roofline"]["per_kernel"].items():
             k["frac_of_streaming_ceiling"] = k["GBps"] / rate["update" if "update" in role else "line"]
 

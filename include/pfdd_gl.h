@@ -112,6 +112,13 @@ typedef struct pfdd_gl_reduce {
 /* Library version, (major << 16) | minor. */
 int pfdd_gl_version(void);
 
+/* Practical HBM streaming rates of GPU `device` (no counterpart in the
+ * reference; context for roofline figures): GB/s of a copy (1 read + 1
+ * write) and of an add (2 reads + 1 write) over arrays of `bytes` each, with
+ * the nontemporal 16-byte accesses of the field passes, averaged over `reps`
+ * launches after 20 untimed ones.  Allocates and frees three arrays. */
+int pfdd_gl_stream_rates(int device, long long bytes, int reps, double *copy_GBps, double *add_GBps);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *pfdd_gl_last_error(void);
 

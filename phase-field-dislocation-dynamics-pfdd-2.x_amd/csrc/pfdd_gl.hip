@@ -261,6 +261,25 @@ __global__ void __launch_bounds__(NTHR)
     if (!finite || (tol > 0 && xinorm < tol)) *done = finite ? 1 : 2;
 }
 
+// Streaming rate probes for pfdd_gl_stream_rates: one 16-byte element per
+// thread, nontemporal loads and stores (the access pattern the field passes
+// use), copy (1 read + 1 write) and add (2 reads + 1 write).
+__global__ void __launch_bounds__(256)
+    k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) st_stream(&b[i], ld_stream(&a[i]));
+}
+__global__ void __launch_bounds__(256)
+    k_stream_add(const double2 *__restrict__ a, const double2 *__restrict__ c, double2 *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const double2 x = ld_stream(&a[i]), y = ld_stream(&c[i]);
+        st_stream(&b[i], make_double2(x.x + y.x, x.y + y.y));
+    }
+}
+
 __global__ void k_reset(int *done, int *steps)
 {
     if (threadIdx.x == 0) {
@@ -1333,6 +1352,53 @@ int self_graph_ready(pfdd_gl *h, double tol)
 extern "C" {
 
 int pfdd_gl_version(void) { return (0 << 16) | 1; }
+
+int pfdd_gl_stream_rates(int device, long long bytes, int reps, double *copy_GBps, double *add_GBps)
+{
+    if (bytes < 4096 || reps < 1 || !copy_GBps || !add_GBps) return fail(PFDD_ERR_ARG, "stream_rates: bad argument");
+    if (hipSetDevice(device) != hipSuccess) return fail(PFDD_ERR_ARG, "stream_rates: no device %d", device);
+    const long long n = bytes / 16;
+    double2 *a = nullptr, *b = nullptr, *c = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PFDD_OK;
+    auto chk = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && !rc) rc = fail(PFDD_ERR_HIP, "stream_rates %s: %s", what, hipGetErrorString(e));
+    };
+    chk(hipMalloc((void **)&a, n * 16), "alloc");
+    if (!rc) chk(hipMalloc((void **)&b, n * 16), "alloc");
+    if (!rc) chk(hipMalloc((void **)&c, n * 16), "alloc");
+    if (!rc) chk(hipStreamCreate(&st), "stream");
+    if (!rc) chk(hipEventCreate(&e0), "event");
+    if (!rc) chk(hipEventCreate(&e1), "event");
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    double rate[2] = {0.0, 0.0};
+    for (int kind = 0; kind < 2 && !rc; ++kind) {
+        auto launch = [&]() {
+            if (kind == 0) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, st, a, b, n);
+            else hipLaunchKernelGGL(k_stream_add, dim3(nb), dim3(256), 0, st, a, c, b, n);
+        };
+        // first touch of the pages, then the clock ramp, then the timing
+        for (int w = 0; w < 20; ++w) launch();
+        chk(hipGetLastError(), "launch");
+        chk(hipEventRecord(e0, st), "record");
+        for (int r = 0; r < reps; ++r) launch();
+        chk(hipEventRecord(e1, st), "record");
+        chk(hipEventSynchronize(e1), "sync");
+        float ms = 0.0f;
+        if (!rc) chk(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (!rc && ms > 0) rate[kind] = (kind == 0 ? 2.0 : 3.0) * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (c) (void)hipFree(c);
+    *copy_GBps = rate[0];
+    *add_GBps = rate[1];
+    return rc;
+}
 
 const char *pfdd_gl_last_error(void) { return g_err.c_str(); }
 

@@ -89,7 +89,7 @@ class CondReduce(ctypes.Structure):
 
 
 EXPORTS = [
-    "pfdd_gl_version", "pfdd_gl_last_error", "pfdd_gl_create", "pfdd_gl_destroy",
+    "pfdd_gl_version", "pfdd_gl_last_error", "pfdd_gl_stream_rates", "pfdd_gl_create", "pfdd_gl_destroy",
     "pfdd_gl_set_xi", "pfdd_gl_get_xi", "pfdd_gl_xi_device", "pfdd_gl_set_xi_device",
     "pfdd_gl_get_xi_device",
     "pfdd_gl_set_tau", "pfdd_gl_set_core_coef", "pfdd_gl_step", "pfdd_gl_xi_sums",
@@ -102,6 +102,16 @@ EXPORTS = [
 ]
 
 _lib = None
+
+
+def stream_rates(nbytes, reps=100, device=0):
+    """GB/s of the library's nontemporal copy and add kernels over arrays of
+    nbytes each (pfdd_gl_stream_rates): the practical streaming rates the
+    field passes are compared with."""
+    L = load_library()
+    cp, ad = np.zeros(1), np.zeros(1)
+    _check(L.pfdd_gl_stream_rates(int(device), int(nbytes), int(reps), _ptr(cp), _ptr(ad)))
+    return float(cp[0]), float(ad[0])
 
 
 def load_library(path: str = LIB_PATH):
@@ -118,6 +128,7 @@ def load_library(path: str = LIB_PATH):
     sig = {
         "pfdd_gl_version": (c_int, []),
         "pfdd_gl_last_error": (ctypes.c_char_p, []),
+        "pfdd_gl_stream_rates": (c_int, [c_int, ctypes.c_longlong, c_int, _dp, _dp]),
         "pfdd_gl_create": (c_int, [ctypes.POINTER(GLDesc), ctypes.POINTER(vp)]),
         "pfdd_gl_destroy": (c_int, [vp]),
         "pfdd_gl_set_xi": (c_int, [vp, _dp]),
