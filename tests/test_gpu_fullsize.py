@@ -122,29 +122,39 @@ def _c1_solver(pb, precision):
     return s
 
 
-def test_c1_fp64_three_steps_match_oracle():
+def test_c1_three_steps_match_oracle():
+    """FP64 within 1e-9 and FP32 within 1e-5 of the FP64 oracle run live for
+    3 iterations at 128^3 x 12 (the largest size the oracle runs in a test):
+    every reduced scalar per iteration and the final field."""
     import pfdd_setup
-    _say("case: 128^3 x 12 fcc production workload, FP64, 3 iterations vs the oracle run live")
+    _say("case: 128^3 x 12 fcc production workload, FP64 and FP32, 3 iterations vs the oracle run live")
     _threads()
     pb = pfdd_setup.fcc_loops_problem((128, 128, 128))
     BB, _, _ = oracle.bmatrix(pb["shape"], pb["C44"], pb["C12"], pb["C11"], pb["dslip"], pb["xn"], pb["xb"],
                               pb["four"])
     st = oracle.GLState(pb["xi0"], BB, pb["cddt"], 3, pb["core_coef"])
     del BB
-    s = _c1_solver(pb, 0)
+    s = {prec: _c1_solver(pb, prec) for prec in (0, 1)}
+    tol = {0: 1e-9, 1: 1e-5}
+    worst = {0: 0.0, 1: 0.0}
     try:
         for it in range(3):
             ref = st.step(pb["tau"])
-            got = s.step(1)
-            for key, gk in (("xinorm", "xinorm"), ("xiave", "xiave"), ("E_el", "e_elastic"), ("E_core", "e_core"),
-                            ("E_stress", "e_stress"), ("E_total", "e_total")):
-                e = abs(got[gk] - ref[key]) / abs(ref[key])
-                assert e < 1e-9, (it, key, got[gk], ref[key])
-        err = relerr(s.get_xi(), st.xi)
-        _say(f"xi rel err after 3 iterations {err:.3e}")
-        assert err < 1e-9
+            for prec in (0, 1):
+                got = s[prec].step(1)
+                for key, gk in (("xinorm", "xinorm"), ("xiave", "xiave"), ("E_el", "e_elastic"),
+                                ("E_core", "e_core"), ("E_stress", "e_stress"), ("E_total", "e_total")):
+                    e = abs(got[gk] - ref[key]) / abs(ref[key])
+                    worst[prec] = max(worst[prec], e)
+                    assert e < tol[prec], (prec, it, key, got[gk], ref[key])
+        for prec in (0, 1):
+            err = relerr(s[prec].get_xi(), st.xi)
+            _say(f"{'FP64' if prec == 0 else 'FP32'}: xi rel err after 3 iterations {err:.3e}, "
+                 f"worst scalar {worst[prec]:.3e}")
+            assert err < tol[prec]
     finally:
-        s.close()
+        for v in s.values():
+            v.close()
 
 
 def test_c1_fp32_1000_steps_vs_fp64():
