@@ -37,9 +37,12 @@ __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__res
 // the tile of the other FP64 cases: at 128^3 x 12 two columns per block (one
 // thread per k-point, no duplicated contraction work, 3 blocks per CU) ran
 // the pass 12 % faster than one column (two threads per k-point, 5 blocks).
+// So do 512-point FP64 lines (two columns of 3 systems instead of one:
+// 512 x 512 x 256 x 3, 2.52 -> 2.27 ms; in FP32 four columns instead of two
+// were slower).
 template <typename R, int N, int S> constexpr int mid_pairs()
 {
-    constexpr int target = sizeof(R) == 8 ? (S == 12 ? 4096 : 2048) : 4096;
+    constexpr int target = sizeof(R) == 8 ? ((S == 12 || N >= 512) ? 4096 : 2048) : 4096;
     int np = 1;
     while (np * 2 * S * N <= target) np *= 2;
     // the padded tile stays within 60 KB of LDS (the constants and the
