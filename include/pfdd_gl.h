@@ -260,7 +260,9 @@ int pfdd_gl_theta(pfdd_gl *h, double *theta);
  * Timing runs on this handle's own buffers: after 20 warm-up iterations, two
  * interleaved rounds of `nsteps` GL iterations per candidate, keeping each
  * kernel's best time; the fastest shape and plan per line pass and the
- * faster x-pass variant win, and variant 2 replaces them if its whole
+ * faster x-pass variant form a mix, which is timed as a whole iteration
+ * against the two uniform candidates with the smallest iteration sums (the
+ * fastest of the three wins), and variant 2 replaces the winner if its whole
  * iteration is faster (choice[5], may be NULL).  The field and step counters
  * are restored; the choice is cached for the process per grid, system count,
  * batch, precision and options.  Tile shapes and iteration variants give the

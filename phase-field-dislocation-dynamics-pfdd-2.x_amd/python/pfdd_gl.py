@@ -279,10 +279,12 @@ class GLSolver:
         return r.as_dict() if sync else None
 
     def autotune(self, nsteps=4):
-        """Pick the fastest LDS tile shape per line pass and the iteration
-        variant; returns the choices (fwd z, fwd y, inv y, inv z + update: 0,
-        1 or 2; iteration: 0 five kernels, 1 five kernels with the pipelined
-        x pass, 2 the three-kernel plane iteration)."""
+        """Pick the fastest LDS tile shape and radix plan per line pass
+        (checked as a whole step) and the iteration variant; returns the
+        choices (fwd z, fwd y, inv y: 0-2, or 3-5 for the same shapes with the
+        radix-16 plan; inv z + update: 0-2; iteration: 0 five kernels, 1 five
+        kernels with the pipelined x pass, 2 the three-kernel plane
+        iteration)."""
         c = (ctypes.c_int * 5)()
         _check(self._lib.pfdd_gl_autotune(self._h, int(nsteps), c))
         return list(c)
