@@ -279,16 +279,74 @@ template <typename R, int N, int TQ> constexpr int tile_lines_u() { return tile_
 
 template <typename R, int N> constexpr int tile_lines() { return tile_lines_q<R, N, 1>(); }
 
+// Addressing of a tile of NL lines: element (position p, line l) at
+// XA::at(p, l), XA::size(n) elements for lines of n.  The Stockham stages
+// address through rd<NBL>(j, r, l) = at(j + r NBL, l) and wr<NS, RAD>(j, r, l)
+// = at((j / NS) NS RAD + j % NS + r NS, l), r a compile-time index, so that a
+// layout can fold the r-dependent part into the instruction's offset.
+// RowPad: rows of NL + 1 elements; the pad puts position-fastest accesses
+// (stride NL + 1, odd) on distinct banks.
+template <int NL> struct RowPad {
+    static constexpr int size(int n) { return n * (NL + 1); }
+    __device__ __forceinline__ static int at(int p, int l) { return p * (NL + 1) + l; }
+    template <int NBL> __device__ __forceinline__ static int rd(int j, int r, int l) { return at(j + r * NBL, l); }
+    template <int NS, int RAD> __device__ __forceinline__ static int wr(int j, int r, int l)
+    {
+        return at((j / NS) * NS * RAD + j % NS + r * NS, l);
+    }
+};
+
+// Swz8: rows of NL (a multiple of 8) elements without the pad, 8 elements
+// of gap after every 16 rows, and the low three bits of the line index
+// XORed with (p >> 1) & 7.  For 8-byte accesses served 16 lanes at a time
+// from 32 four-byte banks, a tile of NL = 24 lines then has no bank
+// conflicts in any of its access orders: line-fastest runs (the FFT stages)
+// are contiguous within each row and the gap separates rows 16 apart (the
+// radix-16 stage writes row 16 j + r next to row 16 (j + 1) + r);
+// position-fastest runs of 16 (loads, stores, the contraction: stride NL)
+// are spread by the XOR.  RowPad's 24-line rows conflict on the 16-lane
+// groups that straddle two rows (one extra cycle per group).  The XOR
+// depends on p mod 16 only, so row offsets that are multiples of 16 fold
+// into constants.
+template <int NL> struct Swz8 {
+    static_assert(NL % 8 == 0, "Swz8 permutes groups of 8 lines");
+    static constexpr int ROW16 = 16 * NL + 8; // 16 rows and their gap
+    static constexpr int size(int n) { return n * NL + ((n + 15) / 16) * 8; }
+    __device__ __forceinline__ static int at(int p, int l) { return p * NL + (p >> 4) * 8 + (l ^ ((p >> 1) & 7)); }
+    template <int NBL> __device__ __forceinline__ static int rd(int j, int r, int l)
+    {
+        if constexpr (NBL % 16 == 0) {
+            return at(j, l) + (r * NBL / 16) * ROW16;
+        } else if constexpr (NBL == 8) {
+            // j < 8: row j + 8 r is row j or j + 8 of block r / 2
+            return at(j + 8 * (r & 1), l) + (r >> 1) * ROW16;
+        } else {
+            return at(j + r * NBL, l);
+        }
+    }
+    template <int NS, int RAD> __device__ __forceinline__ static int wr(int j, int r, int l)
+    {
+        if constexpr (NS % 16 == 0) {
+            return at((j / NS) * NS * RAD + j % NS, l) + (r * NS / 16) * ROW16;
+        } else if constexpr (NS == 1 && RAD == 16) {
+            // row 16 j + r: block j, row r within it
+            return j * ROW16 + r * NL + (l ^ ((r >> 1) & 7));
+        } else {
+            return at((j / NS) * NS * RAD + j % NS + r * NS, l);
+        }
+    }
+};
+
 // One Stockham (auto-sort) radix-RAD stage over all NL lines of the tile,
 // in place in LDS: every thread reads all its inputs, barrier, writes.
 //   input  (line l, j + r*N/RAD), r < RAD, for butterfly j < N/RAD
 //   twiddle w^{r*(j mod NS)}, w = exp(-+2 pi i / (NS*RAD))
 //   output (line l, (j/NS)*NS*RAD + (j mod NS) + r*NS)
-// tw[m] = exp(-2 pi i m / N), m < N.
-template <int N, int RAD, int NS, int NL, int NTHR, bool INV, typename V>
-__device__ __forceinline__ void stockham_stage(const Tile<V> lds, const V *__restrict__ tw, int tid)
+// tw[m] = exp(-2 pi i m / N), m < N.  The arithmetic does not depend on the
+// tile addressing XA.
+template <int N, int RAD, int NS, int NL, int NTHR, bool INV, typename XA, typename V>
+__device__ __forceinline__ void stockham_stage_at(const Tile<V> lds, const V *__restrict__ tw, int tid)
 {
-    constexpr int LDP = NL + 1;
     constexpr int NBL = N / RAD;
     constexpr int NBF = NBL * NL;
     constexpr int PER = (NBF + NTHR - 1) / NTHR;
@@ -299,7 +357,7 @@ __device__ __forceinline__ void stockham_stage(const Tile<V> lds, const V *__res
         if ((NBF % NTHR == 0) || bf < NBF) {
             const int l = bf % NL, j = bf / NL;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) v[q][r] = lds.ld((j + r * NBL) * LDP + l);
+            for (int r = 0; r < RAD; ++r) v[q][r] = lds.ld(XA::template rd<NBL>(j, r, l));
         }
     }
     __syncthreads();
@@ -315,12 +373,17 @@ __device__ __forceinline__ void stockham_stage(const Tile<V> lds, const V *__res
                     v[q][r] = cmulw<INV>(v[q][r], tw[r * k * (N / (NS * RAD))]);
             }
             dft<RAD, INV>(v[q]);
-            const int idxD = (j / NS) * NS * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) lds.st((idxD + r * NS) * LDP + l, v[q][r]);
+            for (int r = 0; r < RAD; ++r) lds.st(XA::template wr<NS, RAD>(j, r, l), v[q][r]);
         }
     }
     __syncthreads();
+}
+
+template <int N, int RAD, int NS, int NL, int NTHR, bool INV, typename V>
+__device__ __forceinline__ void stockham_stage(const Tile<V> lds, const V *__restrict__ tw, int tid)
+{
+    stockham_stage_at<N, RAD, NS, NL, NTHR, INV, RowPad<NL>, V>(lds, tw, tid);
 }
 
 template <int N, int NS, int NL, int NTHR, bool INV, typename V>
@@ -356,15 +419,15 @@ template <int N> constexpr bool radix16_plan_differs() { return stages_x(N) < st
 // thread holds per butterfly (measured 220 -> 199 us FP64, 108 -> 102 us FP32
 // at 128^3 x 12; at 512, where the stage count stays 3, radix 16 only cost
 // registers).  The line passes keep radix 8.
-template <int N, int NS, int NL, int NTHR, bool INV, typename V>
+template <int N, int NS, int NL, int NTHR, bool INV, typename XA = RowPad<NL>, typename V>
 __device__ __forceinline__ void fft_stages_x(const Tile<V> lds, const V *__restrict__ tw, int tid)
 {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix_x(N / NS);
         static_assert(RAD == 2 || RAD == 3 || RAD == 4 || RAD == 5 || RAD == 8 || RAD == 16,
                       "length must be 2^a 3^b 5^c");
-        stockham_stage<N, RAD, NS, NL, NTHR, INV, V>(lds, tw, tid);
-        fft_stages_x<N, NS * RAD, NL, NTHR, INV, V>(lds, tw, tid);
+        stockham_stage_at<N, RAD, NS, NL, NTHR, INV, XA, V>(lds, tw, tid);
+        fft_stages_x<N, NS * RAD, NL, NTHR, INV, XA, V>(lds, tw, tid);
     }
 }
 

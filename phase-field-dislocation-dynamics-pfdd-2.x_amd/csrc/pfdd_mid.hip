@@ -9,11 +9,11 @@ namespace pfdd {
 // load of a thread is issued before its first LDS write, in both precisions
 // (for the 16-byte elements of the FP64 x pass at 128^3 x 12 this measured
 // 191 -> 184 us against a strided loop of 4 loads in flight).
-template <int N, int NL, int NP, int NTHR, typename V>
+template <int N, int NL, int NP, int NTHR, typename XA, typename V>
 __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__restrict__ T2, long long c0, long long ncol,
                                                long long ntot, int tid)
 {
-    constexpr int LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+    constexpr int TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
     {
         V r[PER];
 #pragma unroll
@@ -26,7 +26,7 @@ __device__ __forceinline__ void tile_load_cols(const Tile<V> lds, const V *__res
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
             const int e = tid + t * NTHR;
-            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[t]);
+            if (TOT % NTHR == 0 || e < TOT) lds.st(XA::at(e % N, e / N), r[t]);
         }
     }
 }
@@ -50,6 +50,13 @@ template <typename R, int N, int S> constexpr int mid_pairs()
     while (np > 1 && (long long)N * (S * np + 1) * 2 * (int)sizeof(R) > 61440) np /= 2;
     return np;
 }
+
+// Tile addressing of the on-the-fly x pass: Swz8 for its 24-line FP64 tiles
+// (12 systems x 2 columns at 96 and 128 points), RowPad otherwise.
+template <typename R, int N, int S> struct XLay {
+    static constexpr int NL = S * mid_pairs<R, N, S>();
+    typedef typename std::conditional<(sizeof(R) == 8 && NL == 24 && N % 16 == 0), Swz8<NL>, RowPad<NL>>::type type;
+};
 
 template <typename R, typename RB, int N, int S>
 __global__ void __launch_bounds__(NTHR)
@@ -152,7 +159,7 @@ template <typename T> struct GConst {
 // xihat_a is the value read for the contraction, kept in registers so that
 // the sink does not read the tile again (FP64 x pass at 128^3 x 12: 189 ->
 // 186 us).
-template <typename T, int S, int SA, int H, int NP, int LDP, typename V, typename Sink>
+template <typename T, int S, int SA, int H, int NP, typename XA, typename V, typename Sink>
 __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const T f[3], bool nzf, T c2, T c4,
                                           const T *__restrict__ cst, const GConst<T> &G, Sink sink)
 {
@@ -163,7 +170,7 @@ __device__ __forceinline__ void otf_point(const Tile<V> lds, int p, int q, const
     V xk[SA];
 #pragma unroll
     for (int b = 0; b < S; ++b) {
-        const V v = lds.ld(p * LDP + b * NP + q);
+        const V v = lds.ld(XA::at(p, b * NP + q));
         if (b >= H * SA && b < H * SA + SA) xk[b - H * SA] = v;
         const T xr = (T)v.x, xi = (T)v.y;
 #pragma unroll
@@ -269,11 +276,12 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
     typedef typename cvec<R>::type V;
     typedef R T;
     constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1;
+    constexpr int NL = S * NP;
+    typedef typename XLay<R, N, S>::type XA;
     constexpr int TPP = otf_tpp<R, N, S>();
     constexpr int SA = S / TPP;
     constexpr int NPT = NP * N; // k-points per tile
-    fft_stages_x<N, 1, NL, NTHR, false>(lds, tw, tid);
+    fft_stages_x<N, 1, NL, NTHR, false, XA>(lds, tw, tid);
     // the elastic-energy term Re(conj(xihat_a) ghat_a) is taken as each ghat_a
     // replaces xihat_a in the tile
     double el = 0.0; // a thread handles one or two k-points; the block and grid sums are compensated
@@ -286,8 +294,8 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
             T f[3], c2, c4;
             bool nzf;
             otf_freq<N, NP, S, T, PL>(G, c0 + q, p, f, nzf, c2, c4);
-            otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, [&](int a, T gr, T gi, V x) {
-                const int idx = p * LDP + a * NP + q;
+            otf_point<T, S, SA, 0, NP, XA>(lds, p, q, f, nzf, c2, c4, cst, gc, [&](int a, T gr, T gi, V x) {
+                const int idx = XA::at(p, a * NP + q);
                 el += (double)x.x * gr + (double)x.y * gi;
                 lds.st(idx, mk<V>((R)gr, (R)gi));
             });
@@ -308,15 +316,15 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
         };
         if (ok) {
             if (__builtin_amdgcn_readfirstlane(h) == 0)
-                otf_point<T, S, SA, 0, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, keep);
+                otf_point<T, S, SA, 0, NP, XA>(lds, p, q, f, nzf, c2, c4, cst, gc, keep);
             else
-                otf_point<T, S, SA, 1, NP, LDP>(lds, p, q, f, nzf, c2, c4, cst, gc, keep);
+                otf_point<T, S, SA, 1, NP, XA>(lds, p, q, f, nzf, c2, c4, cst, gc, keep);
         }
         __syncthreads();
         if (ok) {
 #pragma unroll
             for (int aa = 0; aa < SA; ++aa) {
-                const int idx = p * LDP + (h * SA + aa) * NP + q;
+                const int idx = XA::at(p, (h * SA + aa) * NP + q);
                 const V x = lds.ld(idx);
                 el += (double)x.x * g[aa][0] + (double)x.y * g[aa][1];
                 lds.st(idx, mk<V>((R)g[aa][0], (R)g[aa][1]));
@@ -324,7 +332,7 @@ __device__ __forceinline__ double otf_tile(const Tile<typename cvec<R>::type> ld
         }
     }
     __syncthreads();
-    fft_stages_x<N, 1, NL, NTHR, true>(lds, tw, tid);
+    fft_stages_x<N, 1, NL, NTHR, true, XA>(lds, tw, tid);
     return el;
 }
 
@@ -346,8 +354,9 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     typedef typename cvec<R>::type V;
     if (*done) return;
     constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N;
-    PFDD_TILE(V, lds, N * LDP);
+    constexpr int NL = S * NP, TOT = NL * N;
+    typedef typename XLay<R, N, S>::type XA;
+    PFDD_TILE(V, lds, XA::size(N));
     __shared__ R cst[S * 12];
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
@@ -357,14 +366,14 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     T2 += (long long)mem * S * ntot;
     GConst<R> gc;
     otf_consts<R, S>(cst, gc, G, tid);
-    tile_load_cols<N, NL, NP, NTHR>(lds, T2, c0, ncol, ntot, tid);
+    tile_load_cols<N, NL, NP, NTHR, XA>(lds, T2, c0, ncol, ntot, tid);
     __syncthreads();
     double v[1] = {otf_tile<R, N, S>(lds, tw, G, gc, cst, c0, ncol, tid)};
 #pragma unroll 4
     for (int e = tid; e < TOT; e += NTHR) {
         const int l = e / N, p = e % N;
         const int a = l / NP, q = l % NP;
-        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(p * LDP + l));
+        if (c0 + q < ncol) st_stream(&T2[(long long)a * ntot + (c0 + q) * N + p], lds.ld(XA::at(p, l)));
     }
     block_sum_comp<1, NTHR>(v, red, tid);
     if (tid == 0) part_el[blockIdx.x] = v[0];
@@ -389,8 +398,9 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     typedef typename cvec<R>::type V;
     if (*done) return;
     constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
-    PFDD_TILE(V, lds, N * LDP);
+    constexpr int NL = S * NP, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+    typedef typename XLay<R, N, S>::type XA;
+    PFDD_TILE(V, lds, XA::size(N));
     __shared__ R cst[S * 12];
     __shared__ double red[NTHR / 64];
     const int tid = threadIdx.x;
@@ -419,7 +429,7 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
     for (int k = 0; k < PER; ++k) {
         const int e = tid + k * NTHR;
         const int q = e % NP, p = (e / NP) % N, a = e / (NP * N);
-        if (TOT % NTHR == 0 || e < TOT) lds.st(p * LDP + a * NP + q, r[k]);
+        if (TOT % NTHR == 0 || e < TOT) lds.st(XA::at(p, a * NP + q), r[k]);
     }
     __syncthreads();
     double v[1] = {otf_tile<R, N, S, true>(lds, tw, G, gc, cst, c0, ncol, tid)};
@@ -428,7 +438,7 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(otf_w
         const int e = tid + k * NTHR;
         const int q = e % NP, p = (e / NP) % N, a = e / (NP * N);
         if ((TOT % NTHR == 0 || e < TOT) && c0 + q < ncol)
-            T[(long long)a * ntot + (long long)p * ncol + c0 + q] = lds.ld(p * LDP + a * NP + q);
+            T[(long long)a * ntot + (long long)p * ncol + c0 + q] = lds.ld(XA::at(p, a * NP + q));
     }
     block_sum_comp<1, NTHR>(v, red, tid);
     if (tid == 0) part_el[b] = v[0];
@@ -463,8 +473,9 @@ __global__ void __launch_bounds__(NTHR)
     typedef typename cvec<R>::type V;
     if (*done) return;
     constexpr int NP = mid_pairs<R, N, S>();
-    constexpr int NL = S * NP, LDP = NL + 1, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
-    PFDD_TILE(V, lds, N * LDP);
+    constexpr int NL = S * NP, TOT = NL * N, PER = (TOT + NTHR - 1) / NTHR;
+    typedef typename XLay<R, N, S>::type XA;
+    PFDD_TILE(V, lds, XA::size(N));
     __shared__ R cst[S * 12];
     __shared__ double red[NTHR / 64];
     GConst<R> gc;
@@ -488,7 +499,7 @@ __global__ void __launch_bounds__(NTHR)
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int e = tid + k * NTHR;
-            if (TOT % NTHR == 0 || e < TOT) lds.st((e % N) * LDP + e / N, r[k]);
+            if (TOT % NTHR == 0 || e < TOT) lds.st(XA::at(e % N, e / N), r[k]);
         }
         __syncthreads();
         const int tn = t + gridDim.x;
@@ -502,7 +513,7 @@ __global__ void __launch_bounds__(NTHR)
         for (int e = tid; e < TOT; e += NTHR) {
             const int l = e / N, p = e % N;
             const int a = l / NP, q = l % NP;
-            if (c0 + q < ncol) T2m[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(p * LDP + l);
+            if (c0 + q < ncol) T2m[(long long)a * ntot + (c0 + q) * N + p] = lds.ld(XA::at(p, l));
         }
         __syncthreads();
     }
