@@ -171,8 +171,10 @@ int pfdd_gl_set_core_coef(pfdd_gl *h, int ncoef, const double *coef);
  * xinorm < tol (tol <= 0: never).  If `out` is non-NULL the call waits for
  * the GPU and reports the last executed iteration; otherwise it only
  * enqueues the work.  Runs of 20 or more iterations replay a graph of 10
- * iterations (captured once per launch configuration); PFDD_GL_GRAPH=0 in
- * the environment at pfdd_gl_create launches every iteration eagerly. */
+ * iterations (captured once per launch configuration and tol, for both
+ * ping-pong parities together, at the end of autotuning or in the first
+ * long run); PFDD_GL_GRAPH=0 in the environment at pfdd_gl_create launches
+ * every iteration eagerly. */
 int pfdd_gl_step(pfdd_gl *h, int nsteps, double tol, pfdd_gl_reduce *out);
 
 /* Per-system sums of Re xi over all cells of the current field (used by the

@@ -222,6 +222,13 @@ def test_step_graph_replay_equals_single_steps(precision):
         for _ in range(30):
             b.step(1)
         assert np.array_equal(a.get_xi(), b.get_xi())
+        # an odd number of eager iterations switches the ping-pong parity; the
+        # next long run replays the graph captured for the other parity
+        for n in (3, 20, 1, 20):
+            a.step(n)
+            for _ in range(n):
+                b.step(1)
+            assert np.array_equal(a.get_xi(), b.get_xi()), n
     finally:
         a.close()
         b.close()
