@@ -1399,6 +1399,9 @@ int self_graph_capture(pfdd_gl *h, double tol, int par)
     if (e != hipSuccess) return fail(PFDD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     g.graph = graph;
     HIPCHK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    // the graph of the other parity is first launched later, often inside a
+    // timed run: its upload to the device is done here
+    HIPCHK(hipGraphUpload(g.exec, h->stream));
     g.ver = h->version;
     g.tol = tol;
     return PFDD_OK;
