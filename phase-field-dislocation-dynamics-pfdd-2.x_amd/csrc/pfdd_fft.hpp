@@ -284,11 +284,11 @@ template <typename R, int N> constexpr int tile_lines() { return tile_lines_q<R,
 // address through rd<NBL>(j, r, l) = at(j + r NBL, l) and wr<NS, RAD>(j, r, l)
 // = at((j / NS) NS RAD + j % NS + r NS, l), r a compile-time index, so that a
 // layout can fold the r-dependent part into the instruction's offset.
-// RowPad: rows of NL + 1 elements; the pad puts position-fastest accesses
-// (stride NL + 1, odd) on distinct banks.
-template <int NL> struct RowPad {
-    static constexpr int size(int n) { return n * (NL + 1); }
-    __device__ __forceinline__ static int at(int p, int l) { return p * (NL + 1) + l; }
+// RowPad: rows of NL + PAD elements; a pad of 1 puts position-fastest
+// accesses (stride NL + 1, odd) on distinct banks, PAD 0 saves the LDS.
+template <int NL, int PAD = 1> struct RowPad {
+    static constexpr int size(int n) { return n * (NL + PAD); }
+    __device__ __forceinline__ static int at(int p, int l) { return p * (NL + PAD) + l; }
     template <int NBL> __device__ __forceinline__ static int rd(int j, int r, int l) { return at(j + r * NBL, l); }
     template <int NS, int RAD> __device__ __forceinline__ static int wr(int j, int r, int l)
     {

@@ -52,10 +52,14 @@ template <typename R, int N, int S> constexpr int mid_pairs()
 }
 
 // Tile addressing of the on-the-fly x pass: Swz8 for its 24-line FP64 tiles
-// (12 systems x 2 columns at 96 and 128 points), RowPad otherwise.
+// (12 systems x 2 columns at 96 and 128 points); unpadded rows for FP64
+// lines of 512 points, whose padded tile (57.6 KB) leaves room for only two
+// blocks per CU; RowPad otherwise.
 template <typename R, int N, int S> struct XLay {
     static constexpr int NL = S * mid_pairs<R, N, S>();
-    typedef typename std::conditional<(sizeof(R) == 8 && NL == 24 && N % 16 == 0), Swz8<NL>, RowPad<NL>>::type type;
+    typedef typename std::conditional<
+        (sizeof(R) == 8 && NL == 24 && N % 16 == 0), Swz8<NL>,
+        typename std::conditional<(sizeof(R) == 8 && N >= 512), RowPad<NL, 0>, RowPad<NL>>::type>::type type;
 };
 
 template <typename R, typename RB, int N, int S>
@@ -257,12 +261,18 @@ __device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p,
     c4 = (T)(G.inv1mnu * if2 * if2 * G.inv2C44);
 }
 
-// Minimum waves per SIMD asked of the x pass: 5 for the 12-system FP64 cases
-// (their LDS tile allows 5 blocks per CU, and the pass is latency-bound; the
-// register budget of 96 then holds without spills), no constraint otherwise.
+// Minimum waves per SIMD asked of the x pass (one wave per SIMD per resident
+// block of 256 threads).  The 12-system FP64 cases ask for 5, set when their
+// tiles were one column wide; with two-column tiles their LDS allows 3 blocks
+// per CU, which the compiler then reaches (measured, and kept: the register
+// allocation the hint produces is the one timed).  512-point lines ask for
+// what their LDS allows: 3 blocks of the unpadded FP64 tile (49 KB), 5 of
+// the FP32 one (29 KB); without the hint their registers allowed 2 and 4.
 template <typename R, int N, int S> constexpr int otf_waves()
 {
-    return (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) ? 5 : 1;
+    if (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) return 5;
+    if (N >= 512) return sizeof(R) == 8 ? 3 : 5;
+    return 1;
 }
 
 // The transform of one x-pass tile held in LDS (NP columns x S systems):
