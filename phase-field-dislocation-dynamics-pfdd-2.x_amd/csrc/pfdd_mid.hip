@@ -265,13 +265,17 @@ __device__ __forceinline__ void otf_freq(const GParams &G, long long col, int p,
 // block of 256 threads).  The 12-system FP64 cases ask for 5, set when their
 // tiles were one column wide; with two-column tiles their LDS allows 3 blocks
 // per CU, which the compiler then reaches (measured, and kept: the register
-// allocation the hint produces is the one timed).  512-point lines ask for
-// what their LDS allows: 3 blocks of the unpadded FP64 tile (49 KB), 5 of
-// the FP32 one (29 KB); without the hint their registers allowed 2 and 4.
+// allocation the hint produces is the one timed).  The 3-system passes ask
+// for 3 in FP64 (168 instead of 172-190 VGPRs, a few spilled; their tiles
+// allow 3-8 blocks): 3-system batches at 48^3 and 64^3 ran their x pass
+// 80 -> 67 and 193 -> 179 us, and 512-point lines (unpadded tile, 49 KB)
+// 2.25 -> 1.81 ms.  In FP32 only the 512-point lines ask for 5 (1.17 -> 1.11
+// ms); at 48-96 points 5 or 6 waves (96 or 80 VGPRs, with spills) were
+// slower than the compiler's 4.  Other system counts are not constrained.
 template <typename R, int N, int S> constexpr int otf_waves()
 {
     if (S == 12 && sizeof(R) == 8 && N >= 32 && N <= 128) return 5;
-    if (N >= 512) return sizeof(R) == 8 ? 3 : 5;
+    if (S == 3) return sizeof(R) == 8 ? 3 : (N >= 512 ? 5 : 1);
     return 1;
 }
 
