@@ -10,8 +10,17 @@ namespace pfdd {
 // does (it reads data_fftw = xi before the update).  Masked cells keep their
 // value; their xinorm term uses the previous unmasked cell's old value in
 // (a, i, j, k) order (prev[] precomputed on the host, -1 = the initial 0).
+// The FP64 sine-series update on its smallest tile at 32-128 points (8
+// lines at 128 points, 19-20 KB of LDS: 8 blocks per CU) asks for 8 waves per
+// SIMD: 64 instead of 68 VGPRs, 229 -> 224.5 us at 128^3 x 12.  The same
+// hint made the FP32 one 9 % slower.
+template <typename R, int N, int TQ, bool GEN> constexpr int update_waves()
+{
+    return (sizeof(R) == 8 && TQ == 0 && !GEN && N >= 32 && N <= 128) ? 8 : 1;
+}
+
 template <typename R, int N, int TQ, bool GEN>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(update_waves<R, N, TQ, GEN>(), 8)))
     k_update(const typename cvec<R>::type *__restrict__ T1, const typename cvec<R>::type *__restrict__ xo,
              typename cvec<R>::type *__restrict__ xn, const uint8_t *__restrict__ mask,
              const long long *__restrict__ prev, const R *__restrict__ amp, LineMap mi, LineMap mo,
