@@ -121,12 +121,17 @@ def test_skewed_masked_256_z_lines(precision):
     run_pair(case, 12, precision=precision, check_every=4, tol_field=tol, tol_scal=tol)
 
 
+@pytest.mark.parametrize("nx", [64, 96, 128])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_fcc12_64_point_x_pass(precision):
-    """12 systems with 64-point x lines (the x pass at its 12-system tile)."""
-    _say(f"case: 64x16x16 x 12 fcc, 8 steps, precision {precision}")
-    case = P.make_case(shape=(64, 16, 16), systems="fcc12", noise=1e-2, sigma_mu=0.004,
-                       loops=[(0, 2, 8, 32, 8, 5), (7, 0, 20, 8, 8, 4)])
+def test_fcc12_x_pass_tiles(nx, precision):
+    """12 systems with 64-, 96- and 128-point x lines: the x pass at its
+    12-system tiles (48 lines at 64 points; 24 lines at 96 and 128, which in
+    FP64 use the swizzled tile layout, through the generic addressing of the
+    radix-8/4/3 stages at 96 and the folded one of the radix-16/8 stages at
+    128)."""
+    _say(f"case: {nx}x16x16 x 12 fcc, 8 steps, precision {precision}")
+    case = P.make_case(shape=(nx, 16, 16), systems="fcc12", noise=1e-2, sigma_mu=0.004,
+                       loops=[(0, 2, 8, nx // 2, 8, 5), (7, 0, 20, 8, 8, 4)])
     tol = 1e-9 if precision == 0 else 1e-5
     run_pair(case, 8, precision=precision, check_every=4, tol_field=tol, tol_scal=tol)
 
